@@ -1,0 +1,221 @@
+/*
+ * qlten_amd.h -- C ABI of the MI355X (gfx950) block-sparse contraction library.
+ *
+ * The reference (QuantumLiquids/TensorToolkit v0.1.3) has no FFI layer: its
+ * contraction is a header-only C++ template path whose seams are
+ *   - the plan builder      BlockSparseDataTensor::DataBlkGenForTenCtrct
+ *                           (include/qlten/qltensor/blk_spar_data_ten/data_blk_operations.h:411-578)
+ *   - the plan executor     BlockSparseDataTensor::CtrctTwoBSDTAndAssignIn
+ *                           (include/qlten/qltensor/blk_spar_data_ten/global_operations.h:895-992)
+ *                           and CtrctAccordingTask (global_operations.h:994-1080)
+ *   - the memory primitives QLMalloc / QLFree / QLMemcpy (include/qlten/framework/mem_ops.h:38-104)
+ * This ABI sits on those whole-plan seams (never on the per-GEMM
+ * hp_numeric::MatMultiply seam, which would rule out grouping and graph
+ * capture).  The host C++ layer in include/qlten/ keeps the reference's
+ * qlten::Contract / QLTensor<ElemT,QNT> API and calls only these functions.
+ *
+ * Conventions
+ *   - every function returns a qlt_status; QLT_OK == 0.  On failure the
+ *     message is available from qlt_last_error() (thread-local).  Nothing here
+ *     ever calls exit() (the reference exits on CUDA errors, mem_ops.h:28-35).
+ *   - device buffers hold raw block-sparse data exactly as the reference's
+ *     pactual_raw_data_ (blk_spar_data_ten.h:481-487): blocks contiguous in
+ *     ascending block index, each block row-major over its degeneracy shape;
+ *     complex128 is interleaved (re, im) (framework/value_t.h:40,58).
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ */
+#ifndef QLTEN_AMD_H
+#define QLTEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t qlt_status;
+
+enum {
+  QLT_OK = 0,
+  QLT_ERR_INVALID_ARG = 1,
+  QLT_ERR_HIP = 2,
+  QLT_ERR_NOT_INITIALIZED = 3,
+  QLT_ERR_OUT_OF_MEMORY = 4,
+  QLT_ERR_UNSUPPORTED = 5,
+  QLT_ERR_CAPTURE = 6,
+  QLT_ERR_INTERNAL = 7
+};
+
+/* Element types (reference QLTEN_Double / QLTEN_Complex, framework/value_t.h:37-81). */
+enum { QLT_F64 = 0, QLT_C128 = 1 };
+
+/* Arithmetic of the block GEMMs.
+ *   QLT_MODE_NATIVE      FP64 MFMA (complex128 as four real FP64 MFMA products).
+ *   QLT_MODE_SPLIT_BF16  FP64 emulated by exactly scaled multi-term BF16 slices on
+ *                        the BF16 MFMA units, FP32 accumulation over k-chunks short
+ *                        enough to stay exact, FP64 recombination. */
+enum { QLT_MODE_NATIVE = 0, QLT_MODE_SPLIT_BF16 = 1 };
+
+#define QLT_MAX_RANK 8
+
+typedef struct qlt_plan qlt_plan;
+typedef struct qlt_graph qlt_graph;
+
+/* Block metadata of one BlockSparseDataTensor (blk_spar_data_ten.h:53-595).
+ * Blocks must be listed in ascending block index (the std::map order of
+ * blk_idx_data_blk_map_); block index = sum(coords[i] * stride[i]) with
+ * row-major strides over blk_shape (utils_inl.h:125-148). */
+typedef struct {
+  int32_t rank;
+  int64_t n_blks;
+  const int64_t *blk_shape;   /* [rank]           number of QN sectors of each index     */
+  const int64_t *blk_coords;  /* [n_blks * rank]  sector coordinates of each block       */
+  const int64_t *blk_dims;    /* [n_blks * rank]  degeneracy shape of each block         */
+  const int64_t *blk_offsets; /* [n_blks]         element offset of each block (raw data)*/
+} qlt_bsdt_desc;
+
+/* One contraction task; mirrors RawDataCtrctTask
+ * (include/qlten/qltensor/blk_spar_data_ten/raw_data_operation_tasks.h:196-248). */
+typedef struct {
+  int64_t a_blk_idx;
+  int64_t a_data_offset;
+  int64_t b_blk_idx;
+  int64_t b_data_offset;
+  int64_t c_blk_idx;
+  int64_t c_data_offset;
+  int64_t m;
+  int64_t k;
+  int64_t n;
+  double f_ex_sign;
+  double beta;
+} qlt_task;
+
+typedef struct {
+  int64_t n_tasks;       /* number of RawDataCtrctTask records                     */
+  int64_t n_c_blks;      /* number of output blocks (0 for a scalar output)        */
+  int64_t c_raw_size;    /* output raw data size in elements                      */
+  int32_t c_rank;        /* output rank                                            */
+  int32_t c_is_scalar;   /* 1 if every axis was contracted                         */
+  int64_t sum_mkn;       /* sum over tasks of m*k*n (flops = 2x real, 8x complex;
+                            tensor_op_cost.h:384-406)                             */
+  int64_t n_tiles;       /* output tiles of the grouped kernel                     */
+  int64_t a_unique_elems;/* elements of A blocks touched by at least one task      */
+  int64_t b_unique_elems;/* elements of B blocks touched by at least one task      */
+} qlt_plan_info;
+
+/* ---- runtime ---------------------------------------------------------- */
+
+const char *qlt_last_error(void);
+qlt_status qlt_init(int32_t device);
+qlt_status qlt_finalize(void);
+qlt_status qlt_device_sync(void);
+qlt_status qlt_stream_create(void **stream);
+qlt_status qlt_stream_destroy(void *stream);
+qlt_status qlt_stream_sync(void *stream);
+
+/* Device memory behind QLMalloc / QLFree / QLMemcpy (mem_ops.h:38-104). */
+qlt_status qlt_malloc(void **ptr, size_t bytes);
+qlt_status qlt_free(void *ptr);
+qlt_status qlt_memcpy_h2d(void *dst, const void *src, size_t bytes);
+qlt_status qlt_memcpy_d2h(void *dst, const void *src, size_t bytes);
+qlt_status qlt_memcpy_d2d(void *dst, const void *src, size_t bytes);
+qlt_status qlt_memset_zero(void *dst, size_t bytes, void *stream);
+
+/* ---- contraction plan (replaces DataBlkGenForTenCtrct, data_blk_operations.h:411-578) ---
+ * Builds the reference plan on the device with integer kernels: part-hash of
+ * the contracted coordinates (vec_hash.h:39-47, blk_spar_data_ten.h:959-973),
+ * A-major/B-minor matching in ascending block index, output block creation
+ * with beta = 0 on first touch, prefix-sum output offsets in ascending output
+ * block index, c_data_offset fill.  The output index order is
+ * A[a_saved...] ++ B[b_saved...] (ten_ctrct.h:372-394); the general Contract
+ * passes ascending saved axes (data_blk_operations.h:268-296), the contiguous
+ * path passes cyclic ones (contract_contiguous_axes.h:342-353).
+ * The plan also holds the device work list of the grouped GEMM. */
+qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b,
+                          int32_t n_ctrct, const int32_t *a_ctrct_axes,
+                          const int32_t *b_ctrct_axes,
+                          int32_t n_a_saved, const int32_t *a_saved_axes,
+                          int32_t n_b_saved, const int32_t *b_saved_axes,
+                          void *stream, qlt_plan **plan);
+qlt_status qlt_plan_destroy(qlt_plan *plan);
+qlt_status qlt_plan_get_info(const qlt_plan *plan, qlt_plan_info *info);
+/* Tasks in the reference's generation order (A-major, B-minor). */
+qlt_status qlt_plan_get_tasks(const qlt_plan *plan, qlt_task *tasks);
+/* Output block map in ascending block index: blk_idx[n], coords[n*c_rank],
+ * dims[n*c_rank], offsets[n].  Any pointer may be NULL. */
+qlt_status qlt_plan_get_c_blocks(const qlt_plan *plan, int64_t *blk_idx,
+                                 int64_t *coords, int64_t *dims, int64_t *offsets);
+/* Point the plan's output offsets at an existing output layout (the
+ * accumulate path, contract_contiguous_axes.h:389-470): c_offsets[i] is the
+ * element offset, in the caller's output buffer, of the plan's i-th output
+ * block (ascending block index). */
+qlt_status qlt_plan_set_c_offsets(qlt_plan *plan, const int64_t *c_offsets);
+/* Allocate the work buffers an execute in `mode` needs, so that a later
+ * execute allocates nothing (required before stream capture). */
+qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode);
+
+/* ---- execution (replaces CtrctTwoBSDTAndAssignIn, global_operations.h:895-992) ---
+ * For every output block Cb of the plan:
+ *     Cb = alpha * sum_{tasks t -> Cb} A'_t (m x k) * B'_t (k x n) + beta * Cb
+ * where A'_t is A's block permuted to (saved ++ contracted) and B'_t is B's
+ * block permuted to (contracted ++ saved), both row-major -- the transposes of
+ * global_operations.h:919-967 are folded into the kernel's gather prologue.
+ * beta == 0 never reads C (C may hold garbage, as after the reference's
+ * non-zeroing Allocate).  alpha/beta point to 1 (f64) or 2 (c128: re, im)
+ * doubles.  a, b, c are device pointers to the raw data buffers. */
+qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mode,
+                                const void *a, const void *b, void *c,
+                                const double *alpha, const double *beta,
+                                void *stream);
+
+/* ---- graph capture of a repeated contraction chain (Lanczos / DMRG sweep) ---- */
+qlt_status qlt_graph_begin(void *stream);
+qlt_status qlt_graph_end(void *stream, qlt_graph **graph);
+qlt_status qlt_graph_replay(qlt_graph *graph, void *stream);
+qlt_status qlt_graph_destroy(qlt_graph *graph);
+
+/* ---- raw-data elementwise primitives --------------------------------------
+ * Seeded fill used for inputs: element i (real: value i; complex: re = value
+ * 2i, im = value 2i+1) is 2u-1 with u = (SplitMix64(seed + (i+1)*0x9E3779B97F4A7C15) >> 11) * 2^-53.
+ * It replaces the reference's curand/mt19937 Random (raw_data_operations.h:141-166),
+ * whose stream is implementation-defined. */
+qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t first_index,
+                            void *stream);
+/* y = a*x + b*y over n elements (LinearCombine building block, ten_linear_combine.h:186). */
+qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x,
+                     const double *b, void *y, void *stream);
+/* out[0..1] = sum conj(x_i) * y_i (InnerProduct, basic_operations.h:203); real: out[1]=0.
+ * `out` is a host pointer; the call synchronises the stream. */
+qlt_status qlt_dot(int32_t dtype, int64_t n, const void *x, const void *y, double *out,
+                   void *stream);
+/* Real -> complex promotion (VectorRealToCplxKernel, framework/hp_numeric/blas_level1.h:825-843). */
+qlt_status qlt_real_to_complex(int64_t n, const double *x, double *z, void *stream);
+/* Complex conjugation in place (ConjugateKernel, utility/utils_inl.h:556-571). */
+qlt_status qlt_conjugate(int64_t n, double *z, void *stream);
+/* Scale selected blocks: x[off_i .. off_i+len_i) *= s  (untouched-block scaling of the
+ * accumulate path, contract_contiguous_axes.h ScaleUntouchedOutputBlocks_). */
+qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,
+                              const int64_t *lengths, const double *s, void *x, void *stream);
+
+/* ---- batched FP64 block decompositions (replaces MatSVD / MatQR, lapack.h:41-620,637-1916) ----
+ * n_mats dense row-major matrices, matrix i has rows[i] x cols[i] doubles at
+ * a + a_offsets[i] (device).  SVD: A = U diag(S) Vt with U (rows x r), S (r),
+ * Vt (r x cols), r = min(rows, cols), singular values descending, written at
+ * u + u_offsets[i], s + s_offsets[i], vt + vt_offsets[i].  QR: A = Q R with
+ * Q (rows x r), R (r x cols).  All offsets are host arrays in elements. */
+qlt_status qlt_block_svd_f64(int64_t n_mats, const int64_t *rows, const int64_t *cols,
+                             const double *a, const int64_t *a_offsets,
+                             double *u, const int64_t *u_offsets,
+                             double *s, const int64_t *s_offsets,
+                             double *vt, const int64_t *vt_offsets, void *stream);
+qlt_status qlt_block_qr_f64(int64_t n_mats, const int64_t *rows, const int64_t *cols,
+                            const double *a, const int64_t *a_offsets,
+                            double *q, const int64_t *q_offsets,
+                            double *r, const int64_t *r_offsets, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QLTEN_AMD_H */

@@ -1,0 +1,141 @@
+// Internal declarations shared by the HIP translation units of libqlten_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qlten_amd.h"
+
+namespace qlt {
+
+// ---- error reporting ------------------------------------------------------
+void set_error(const std::string &msg);
+qlt_status fail(qlt_status code, const std::string &msg);
+qlt_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+bool initialized();
+int current_device();
+
+#define QLT_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t qlt_e_ = (call);                                         \
+    if (qlt_e_ != hipSuccess)                                           \
+      return ::qlt::hip_fail(qlt_e_, #call, __FILE__, __LINE__);        \
+  } while (0)
+
+#define QLT_CHECK_ARG(cond, msg)                                        \
+  do {                                                                  \
+    if (!(cond)) return ::qlt::fail(QLT_ERR_INVALID_ARG, (msg));        \
+  } while (0)
+
+#define QLT_TRY(expr)                                                   \
+  do {                                                                  \
+    qlt_status qlt_s_ = (expr);                                         \
+    if (qlt_s_ != QLT_OK) return qlt_s_;                                \
+  } while (0)
+
+#define QLT_REQUIRE_INIT()                                              \
+  do {                                                                  \
+    if (!::qlt::initialized())                                          \
+      return ::qlt::fail(QLT_ERR_NOT_INITIALIZED,                       \
+                         "qlt_init() has not been called");             \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// True when `s` is currently being captured into a graph.
+bool stream_capturing(hipStream_t s);
+
+// ---- device work-list records -------------------------------------------
+
+// Gather descriptor of one input block: the block permuted to
+// (row axes ++ col axes), each part decomposed row-major, last axis fastest.
+// For A the rows are the saved axes and the cols the contracted axes; for B
+// the rows are the contracted axes and the cols the saved axes.
+struct BlkGather {
+  int32_t rdim[QLT_MAX_RANK];
+  int32_t rstr[QLT_MAX_RANK];
+  int32_t cdim[QLT_MAX_RANK];
+  int32_t cstr[QLT_MAX_RANK];
+};
+
+// One task in execution order (sorted by output block, generation order
+// inside a block).
+struct ExecTask {
+  int64_t a_off;   // element offset of the A block
+  int64_t b_off;   // element offset of the B block
+  int32_t a_blk;   // position of the A block in A's block list
+  int32_t b_blk;   // position of the B block in B's block list
+  int32_t k;
+  int32_t pad;
+};
+
+// One output block (segment of consecutive ExecTasks).
+struct Seg {
+  int64_t c_off;
+  int32_t m;
+  int32_t n;
+  int32_t t0;      // first ExecTask
+  int32_t t1;      // one past the last ExecTask
+  int64_t k_total; // sum of k over the segment's tasks
+};
+
+// One output tile of the grouped kernel.
+struct Tile {
+  int32_t seg;
+  int32_t tm;      // tile row index within the block
+  int32_t tn;      // tile column index within the block
+  int32_t cls;     // tile-size class
+};
+
+// Axis-split layout of a plan (same permutation for every block pair).
+struct GatherLayout {
+  int32_t a_nr, a_nc;  // A: saved count, contracted count
+  int32_t b_nr, b_nc;  // B: contracted count, saved count
+  int32_t a_kfast;     // A's innermost memory axis is its last contracted axis
+  int32_t b_nfast;     // B's innermost memory axis is its last saved axis
+};
+
+}  // namespace qlt
+
+// Opaque plan (definition shared between plan.hip and the GEMM units).
+struct qlt_plan {
+  int device = 0;
+  // ---- host copy of the reference plan ----
+  std::vector<qlt_task> tasks;          // generation order
+  std::vector<int64_t> c_blk_idx;       // ascending
+  std::vector<int64_t> c_coords;        // n_c * c_rank
+  std::vector<int64_t> c_dims;          // n_c * c_rank
+  std::vector<int64_t> c_offsets;       // n_c
+  int64_t c_raw_size = 0;
+  int32_t c_rank = 0;
+  int32_t c_is_scalar = 0;
+  int64_t sum_mkn = 0;
+  int64_t a_unique_elems = 0;
+  int64_t b_unique_elems = 0;
+  int64_t max_blk_elems_a = 0;
+  int64_t max_blk_elems_b = 0;
+  int64_t a_raw_extent = 0;             // max over A blocks of offset + size
+  int64_t b_raw_extent = 0;
+  // ---- device work list ----
+  qlt::GatherLayout layout{};
+  int64_t n_exec = 0;                   // == tasks.size()
+  int64_t n_segs = 0;
+  int64_t n_tiles = 0;
+  int64_t n_a_blks = 0;
+  int64_t n_b_blks = 0;
+  qlt::ExecTask *d_exec = nullptr;
+  qlt::Seg *d_segs = nullptr;
+  qlt::Tile *d_tiles = nullptr;
+  qlt::BlkGather *d_ga = nullptr;
+  qlt::BlkGather *d_gb = nullptr;
+  std::vector<qlt::Seg> h_segs;         // host mirror (used to rebuild offsets)
+  // tile-size class table chosen at build time
+  int32_t tile_m = 64;
+  int32_t tile_n = 64;
+  // ---- split-precision workspace ----
+  void *d_work = nullptr;
+  size_t work_bytes = 0;
+};

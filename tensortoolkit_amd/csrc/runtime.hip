@@ -1,0 +1,454 @@
+// Runtime part of libqlten_amd: error state, device selection, memory,
+// streams, graph capture and the raw-data elementwise primitives.
+//
+// Error convention: every entry point returns a status and records a
+// thread-local message; nothing exits the process (the reference prints and
+// calls std::exit on CUDA errors, include/qlten/framework/mem_ops.h:28-35).
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "qlt_internal.h"
+
+namespace qlt {
+
+namespace {
+thread_local std::string g_last_error;
+bool g_initialized = false;
+int g_device = 0;
+double *g_reduce_buf = nullptr;          // partial sums of qlt_dot
+constexpr int kReduceBlocks = 1024;
+std::mutex g_init_mutex;
+}  // namespace
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+qlt_status fail(qlt_status code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+qlt_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", static_cast<int>(e),
+                hipGetErrorString(e), what, file, line);
+  g_last_error = buf;
+  return e == hipErrorOutOfMemory ? QLT_ERR_OUT_OF_MEMORY : QLT_ERR_HIP;
+}
+
+bool initialized() { return g_initialized; }
+int current_device() { return g_device; }
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st == hipStreamCaptureStatusActive;
+}
+
+}  // namespace qlt
+
+using namespace qlt;
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double splitmix_uniform(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const double u = static_cast<double>(z >> 11) * 0x1.0p-53;
+  return 2.0 * u - 1.0;
+}
+
+__global__ void k_fill_uniform(double *x, int64_t n, uint64_t seed, int64_t first) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    x[i] = splitmix_uniform(seed, static_cast<uint64_t>(first + i));
+}
+
+__global__ void k_zero(uint8_t *p, int64_t bytes) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int64_t nwords = bytes / 8;
+  uint64_t *w = reinterpret_cast<uint64_t *>(p);
+  for (int64_t j = i; j < nwords; j += stride) w[j] = 0;
+  for (int64_t j = nwords * 8 + i; j < bytes; j += stride) p[j] = 0;
+}
+
+__global__ void k_axpby_real(int64_t n, double a, const double *x, double b, double *y) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
+}
+
+__global__ void k_axpby_cplx(int64_t n, double ar, double ai, const double *x, double br,
+                             double bi, double *y) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const bool bzero = (br == 0.0 && bi == 0.0);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    double yr = ar * xr - ai * xi, yi = ar * xi + ai * xr;
+    if (!bzero) {
+      const double vr = y[2 * i], vi = y[2 * i + 1];
+      yr += br * vr - bi * vi;
+      yi += br * vi + bi * vr;
+    }
+    y[2 * i] = yr;
+    y[2 * i + 1] = yi;
+  }
+}
+
+// Two-level reduction of sum conj(x) * y: block partials, then one block.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double *x, const double *y,
+                                                     double *partial) {
+  __shared__ double sr[256], si[256];
+  double accr = 0.0, acci = 0.0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    if (CPLX) {
+      const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+      accr += xr * yr + xi * yi;
+      acci += xr * yi - xi * yr;
+    } else {
+      accr += x[i] * y[i];
+    }
+  }
+  sr[threadIdx.x] = accr;
+  si[threadIdx.x] = acci;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sr[0];
+    partial[2 * blockIdx.x + 1] = si[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dot_final(int nparts, double *partial) {
+  __shared__ double sr[256], si[256];
+  double accr = 0.0, acci = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    accr += partial[2 * i];
+    acci += partial[2 * i + 1];
+  }
+  sr[threadIdx.x] = accr;
+  si[threadIdx.x] = acci;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * kReduceBlocks] = sr[0];
+    partial[2 * kReduceBlocks + 1] = si[0];
+  }
+}
+
+__global__ void k_real_to_complex(int64_t n, const double *x, double *z) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    z[2 * i] = x[i];
+    z[2 * i + 1] = 0.0;
+  }
+}
+
+__global__ void k_conjugate(int64_t n, double *z) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    z[2 * i + 1] = -z[2 * i + 1];
+}
+
+// One block per segment; scale by a real or complex factor.
+template <bool CPLX>
+__global__ void k_scale_segments(const int64_t *offs, const int64_t *lens, double sr, double si,
+                                 double *x) {
+  const int64_t off = offs[blockIdx.x], len = lens[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
+    if (CPLX) {
+      double *p = x + 2 * (off + i);
+      const double r = p[0], im = p[1];
+      p[0] = sr * r - si * im;
+      p[1] = sr * im + si * r;
+    } else {
+      x[off + i] *= sr;
+    }
+  }
+}
+
+inline unsigned grid_for(int64_t n, int threads = 256, int64_t cap = 8192) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *qlt_last_error(void) { return g_last_error.c_str(); }
+
+qlt_status qlt_init(int32_t device) {
+  std::lock_guard<std::mutex> lock(g_init_mutex);
+  int count = 0;
+  QLT_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(QLT_ERR_INVALID_ARG, "qlt_init: device " + std::to_string(device) +
+                                         " out of range (" + std::to_string(count) + " devices)");
+  QLT_HIP(hipSetDevice(device));
+  if (g_initialized && g_device == device) return QLT_OK;
+  if (g_reduce_buf != nullptr) {
+    (void)hipFree(g_reduce_buf);
+    g_reduce_buf = nullptr;
+  }
+  QLT_HIP(hipMalloc(&g_reduce_buf, sizeof(double) * (2 * kReduceBlocks + 2)));
+  g_device = device;
+  g_initialized = true;
+  return QLT_OK;
+}
+
+qlt_status qlt_finalize(void) {
+  std::lock_guard<std::mutex> lock(g_init_mutex);
+  if (!g_initialized) return QLT_OK;
+  QLT_HIP(hipDeviceSynchronize());
+  if (g_reduce_buf != nullptr) QLT_HIP(hipFree(g_reduce_buf));
+  g_reduce_buf = nullptr;
+  g_initialized = false;
+  return QLT_OK;
+}
+
+qlt_status qlt_device_sync(void) {
+  QLT_REQUIRE_INIT();
+  QLT_HIP(hipDeviceSynchronize());
+  return QLT_OK;
+}
+
+qlt_status qlt_stream_create(void **stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(stream != nullptr, "qlt_stream_create: null output");
+  hipStream_t s;
+  QLT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return QLT_OK;
+}
+
+qlt_status qlt_stream_destroy(void *stream) {
+  QLT_REQUIRE_INIT();
+  if (stream != nullptr) QLT_HIP(hipStreamDestroy(as_stream(stream)));
+  return QLT_OK;
+}
+
+qlt_status qlt_stream_sync(void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_HIP(hipStreamSynchronize(as_stream(stream)));
+  return QLT_OK;
+}
+
+qlt_status qlt_malloc(void **ptr, size_t bytes) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(ptr != nullptr, "qlt_malloc: null output");
+  *ptr = nullptr;
+  if (bytes == 0) return QLT_OK;
+  QLT_HIP(hipMalloc(ptr, bytes));
+  return QLT_OK;
+}
+
+qlt_status qlt_free(void *ptr) {
+  QLT_REQUIRE_INIT();
+  if (ptr != nullptr) QLT_HIP(hipFree(ptr));
+  return QLT_OK;
+}
+
+qlt_status qlt_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  QLT_REQUIRE_INIT();
+  if (bytes == 0) return QLT_OK;
+  QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return QLT_OK;
+}
+
+qlt_status qlt_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  QLT_REQUIRE_INIT();
+  if (bytes == 0) return QLT_OK;
+  QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return QLT_OK;
+}
+
+qlt_status qlt_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  QLT_REQUIRE_INIT();
+  if (bytes == 0) return QLT_OK;
+  QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return QLT_OK;
+}
+
+qlt_status qlt_memset_zero(void *dst, size_t bytes, void *stream) {
+  QLT_REQUIRE_INIT();
+  if (bytes == 0) return QLT_OK;
+  QLT_CHECK_ARG(dst != nullptr, "qlt_memset_zero: null buffer");
+  // A fill kernel rather than hipMemsetAsync, so the call is safe inside a
+  // captured graph.
+  const int64_t n = static_cast<int64_t>(bytes);
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n / 8 + 1)), dim3(256), 0, as_stream(stream),
+                     static_cast<uint8_t *>(dst), n);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+struct qlt_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+qlt_status qlt_graph_begin(void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(stream != nullptr, "qlt_graph_begin: capture needs a non-null stream");
+  QLT_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+  return QLT_OK;
+}
+
+qlt_status qlt_graph_end(void *stream, qlt_graph **graph) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(graph != nullptr, "qlt_graph_end: null output");
+  hipGraph_t g = nullptr;
+  QLT_HIP(hipStreamEndCapture(as_stream(stream), &g));
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  auto *out = new qlt_graph;
+  out->graph = g;
+  out->exec = ge;
+  *graph = out;
+  return QLT_OK;
+}
+
+qlt_status qlt_graph_replay(qlt_graph *graph, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(graph != nullptr && graph->exec != nullptr, "qlt_graph_replay: null graph");
+  QLT_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
+  return QLT_OK;
+}
+
+qlt_status qlt_graph_destroy(qlt_graph *graph) {
+  if (graph == nullptr) return QLT_OK;
+  if (graph->exec != nullptr) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph != nullptr) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return QLT_OK;
+}
+
+qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t first_index,
+                            void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(n_doubles >= 0 && first_index >= 0, "qlt_fill_uniform: negative size");
+  if (n_doubles == 0) return QLT_OK;
+  QLT_CHECK_ARG(x != nullptr, "qlt_fill_uniform: null buffer");
+  hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(n_doubles)), dim3(256), 0, as_stream(stream),
+                     static_cast<double *>(x), n_doubles, seed, first_index);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x, const double *b,
+                     void *y, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_axpby: bad dtype");
+  QLT_CHECK_ARG(a != nullptr && b != nullptr, "qlt_axpby: null scalar");
+  if (n <= 0) return QLT_OK;
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL(k_axpby_real, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, a[0],
+                       static_cast<const double *>(x), b[0], static_cast<double *>(y));
+  else
+    hipLaunchKernelGGL(k_axpby_cplx, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, a[0],
+                       a[1], static_cast<const double *>(x), b[0], b[1], static_cast<double *>(y));
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_dot(int32_t dtype, int64_t n, const void *x, const void *y, double *out,
+                   void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_dot: bad dtype");
+  QLT_CHECK_ARG(out != nullptr, "qlt_dot: null output");
+  out[0] = out[1] = 0.0;
+  if (n <= 0) return QLT_OK;
+  hipStream_t s = as_stream(stream);
+  const unsigned g = std::min<unsigned>(grid_for(n), kReduceBlocks);
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL(k_dot_partial<false>, dim3(g), dim3(256), 0, s, n,
+                       static_cast<const double *>(x), static_cast<const double *>(y), g_reduce_buf);
+  else
+    hipLaunchKernelGGL(k_dot_partial<true>, dim3(g), dim3(256), 0, s, n,
+                       static_cast<const double *>(x), static_cast<const double *>(y), g_reduce_buf);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, static_cast<int>(g), g_reduce_buf);
+  QLT_HIP(hipGetLastError());
+  QLT_HIP(hipMemcpyAsync(out, g_reduce_buf + 2 * kReduceBlocks, 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  QLT_HIP(hipStreamSynchronize(s));
+  return QLT_OK;
+}
+
+qlt_status qlt_real_to_complex(int64_t n, const double *x, double *z, void *stream) {
+  QLT_REQUIRE_INIT();
+  if (n <= 0) return QLT_OK;
+  QLT_CHECK_ARG(x != nullptr && z != nullptr, "qlt_real_to_complex: null buffer");
+  hipLaunchKernelGGL(k_real_to_complex, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, x, z);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_conjugate(int64_t n, double *z, void *stream) {
+  QLT_REQUIRE_INIT();
+  if (n <= 0) return QLT_OK;
+  QLT_CHECK_ARG(z != nullptr, "qlt_conjugate: null buffer");
+  hipLaunchKernelGGL(k_conjugate, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, z);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,
+                              const int64_t *lengths, const double *s, void *x, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_scale_segments: bad dtype");
+  QLT_CHECK_ARG(s != nullptr, "qlt_scale_segments: null scalar");
+  if (n_segs <= 0) return QLT_OK;
+  QLT_CHECK_ARG(offsets != nullptr && lengths != nullptr && x != nullptr,
+                "qlt_scale_segments: null array");
+  hipStream_t st = as_stream(stream);
+  QLT_CHECK_ARG(!stream_capturing(st), "qlt_scale_segments: not capturable (stages host arrays)");
+  int64_t *d = nullptr;
+  QLT_HIP(hipMallocAsync(reinterpret_cast<void **>(&d), 2 * n_segs * sizeof(int64_t), st));
+  QLT_HIP(hipMemcpyAsync(d, offsets, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  QLT_HIP(hipMemcpyAsync(d + n_segs, lengths, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL(k_scale_segments<false>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
+                       d, d + n_segs, s[0], 0.0, static_cast<double *>(x));
+  else
+    hipLaunchKernelGGL(k_scale_segments<true>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
+                       d, d + n_segs, s[0], s[1], static_cast<double *>(x));
+  QLT_HIP(hipGetLastError());
+  QLT_HIP(hipStreamSynchronize(st));
+  QLT_HIP(hipFreeAsync(d, st));
+  QLT_HIP(hipStreamSynchronize(st));
+  return QLT_OK;
+}
+
+}  // extern "C"

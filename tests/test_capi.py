@@ -1,0 +1,47 @@
+"""The C-ABI library loads and exports every function include/qlten_amd.h declares."""
+import ctypes
+import re
+from pathlib import Path
+
+from tensortoolkit_amd import capi
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    text = (ROOT / "include" / "qlten_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qlt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    assert "qlt_plan_build" in names and "qlt_contract_execute" in names
+    assert len(names) >= 30
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(capi.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert sorted(capi.EXPORTED_SYMBOLS) == declared_functions()
+
+
+def test_task_record_layout_matches_reference_fields():
+    # RawDataCtrctTask (raw_data_operation_tasks.h:196-208): nine size_t fields
+    # then f_ex_sign and beta.
+    assert capi.TASK_DTYPE.names[:9] == ("a_blk_idx", "a_data_offset", "b_blk_idx", "b_data_offset",
+                                        "c_blk_idx", "c_data_offset", "m", "k", "n")
+    assert capi.TASK_DTYPE.itemsize == 11 * 8
+
+
+def test_calls_before_init_report_an_error():
+    lib = capi.lib()
+    p = ctypes.c_void_p()
+    # qlt_malloc without qlt_init returns QLT_ERR_NOT_INITIALIZED (3) and sets a message
+    if capi._initialized_device is None:
+        assert lib.qlt_malloc(ctypes.byref(p), 16) == 3
+        assert b"qlt_init" in lib.qlt_last_error()

@@ -1,0 +1,162 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is compared with it.
+
+Plan: brute-force enumeration of block pairs by coordinates (no hashing),
+the reference's cost known-answer test and the closed form of the rank-3 U(1)
+pair.  Values: dense equivalence against numpy tensordot (the reference's
+own check, test_ten_ctrct.cc:226-494) and the closed forms t.t^dagger and
+full contraction (test_ten_ctrct.cc:187-206, :272-306).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tensortoolkit_amd import workloads as W
+from tensortoolkit_amd.blocksparse import IN, OUT, BlockSparseTensor, Index, splitmix_uniform
+from tests.helpers import random_case, rel_frobenius
+
+P1 = 11400714785074694791
+P2 = 14029467366897019727
+P5 = 2870177450012600261
+M64 = (1 << 64) - 1
+
+
+def test_part_hash_empty_and_single():
+    # VecPartHasher with no selected axes is P5 + (0 ^ P5) (mod 2^64);
+    # one coordinate c gives rotl31(P5 + c*P2) * P1 + (1 ^ P5).
+    assert O.part_hash([3, 4], []) == (P5 + P5) & M64
+    c = 5
+    h = (P5 + c * P2) & M64
+    h = ((h << 31) | (h >> 33)) & M64
+    h = (h * P1) & M64
+    h = (h + (1 ^ P5)) & M64
+    assert O.part_hash([9, c], [1]) == h
+
+
+def test_part_hash_axis_order_matters():
+    # The hash is taken over the contracted coordinates in the given axis
+    # order, so swapping the axis list changes it unless the coordinates agree.
+    assert O.part_hash([1, 2], [0, 1]) != O.part_hash([1, 2], [1, 0])
+    assert O.part_hash([2, 2], [0, 1]) == O.part_hash([2, 2], [1, 0])
+
+
+def test_fill_matches_numpy():
+    np.testing.assert_array_equal(O.fill_uniform(1000, 42, 5), splitmix_uniform(1000, 42, 5))
+    x = splitmix_uniform(100000, 1)
+    assert x.min() >= -1.0 and x.max() < 1.0 and abs(x.mean()) < 0.01
+
+
+def test_cost_known_answer():
+    """tests/test_tensor_manipulation/test_tensor_op_cost.cc:19-42."""
+    qn0 = ((0,),)
+    lhs = BlockSparseTensor.allowed_blocks([Index(qn0, (2,), OUT), Index(qn0, (3,), OUT)], (0,))
+    rhs = BlockSparseTensor.allowed_blocks([Index(qn0, (3,), IN), Index(qn0, (4,), OUT)], (0,))
+    lhs.data = np.ones(lhs.raw_size)
+    rhs.data = np.ones(rhs.raw_size)
+    p = O.OraclePlan(lhs, rhs, [1], [0])
+    t = p.tasks()
+    assert p.n_tasks == 1 and p.n_c == 1 and p.c_raw_size == 8
+    flops = 2.0 * float((t["m"] * t["k"] * t["n"]).sum())
+    assert flops == 48.0
+    assert (t["m"][0] * t["k"][0] + t["k"][0] * t["n"][0]) == 18     # read elements
+    _, _, dims, _ = p.c_blocks()
+    assert dims.tolist() == [[2, 4]]
+    np.testing.assert_array_equal(p.execute(), np.full(8, 3.0))
+
+
+def test_rank3_u1_closed_form():
+    """Rank-3 U(1) pair: 17 blocks each, 32 tasks of (16,16,16), 32 output blocks."""
+    a, b, ax = W.config0()
+    assert a.n_blks == 17 and b.n_blks == 17
+    p = O.OraclePlan(a, b, *ax)
+    t = p.tasks()
+    assert p.n_tasks == 32 and p.n_c == 32
+    assert set(zip(t["m"], t["k"], t["n"])) == {(16, 16, 16)}
+    assert 2 * int((t["m"] * t["k"] * t["n"]).sum()) == 262144
+    assert (t["beta"] == 0).all()     # every output block is touched once
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("nq", [1, 2])
+def test_plan_matches_brute_force(seed, nq):
+    rng = np.random.default_rng(1000 + seed + 100 * nq)
+    a, b, a_ax, b_ax = random_case(rng, nq=nq)
+    p = O.OraclePlan(a, b, a_ax, b_ax)
+    tasks = p.tasks()
+    bf = O.brute_force_plan(a, b, a_ax, b_ax)
+    assert len(tasks) == len(bf)
+    a_idx, b_idx = a.blk_idx, b.blk_idx
+    seen = set()
+    c_idx, c_coords, c_dims, c_offs = p.c_blocks()
+    coord_of = {int(i): tuple(c) for i, c in zip(c_idx, c_coords)}
+    for t, (i, j, cc, m, k, n) in zip(tasks, bf):
+        assert t["a_blk_idx"] == a_idx[i] and t["b_blk_idx"] == b_idx[j]
+        assert t["a_data_offset"] == a.offsets[i] and t["b_data_offset"] == b.offsets[j]
+        assert (t["m"], t["k"], t["n"]) == (m, k, n)
+        if cc:
+            assert coord_of[int(t["c_blk_idx"])] == cc
+            assert t["beta"] == (1.0 if cc in seen else 0.0)
+            seen.add(cc)
+    if len(bf) and len(bf[0][2]):
+        # output offsets: prefix sums over ascending block index
+        assert list(c_idx) == sorted(c_idx)
+        sizes = np.prod(c_dims, axis=1)
+        np.testing.assert_array_equal(c_offs, np.concatenate([[0], np.cumsum(sizes)[:-1]]))
+        assert p.c_raw_size == int(sizes.sum())
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_values_match_dense(seed, dtype):
+    rng = np.random.default_rng(2000 + seed)
+    a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1 + seed % 2)
+    p = O.OraclePlan(a, b, a_ax, b_ax)
+    c = p.output_tensor(dtype)
+    c.data = p.execute()
+    dense = O.dense_contract(a, b, a_ax, b_ax)
+    if p.c_rank == 0:
+        got = c.data[0] if p.c_raw_size else 0.0
+        assert abs(got - dense) <= 1e-13 * max(1.0, abs(dense))
+    else:
+        assert rel_frobenius(c.to_dense(), dense) <= 1e-13 or np.linalg.norm(dense) == 0
+
+
+def test_generation_order_execution_agrees():
+    rng = np.random.default_rng(7)
+    a, b, a_ax, b_ax = random_case(rng, allow_scalar=False, max_deg=6)
+    p = O.OraclePlan(a, b, a_ax, b_ax)
+    assert rel_frobenius(p.execute(False), p.execute(True)) < 1e-14
+
+
+def test_t_tdagger_closed_form():
+    """test_ten_ctrct.cc:187-206: rank-1 t contracted with its conjugate = sum |t_i|^2."""
+    idx = Index(((-1,), (0,), (1,)), (3, 10, 3), OUT)
+    t = BlockSparseTensor.allowed_blocks([idx], (0,), "c128").fill_uniform(5)
+    td = BlockSparseTensor(
+        [idx.inverse()], t.coords, t.dims, "c128", data=np.conj(t.data))
+    p = O.OraclePlan(t, td, [0], [0])
+    got = p.execute()[0]
+    assert abs(got - np.sum(np.abs(t.data) ** 2)) < 1e-13 * abs(got)
+    assert abs(got.imag) < 1e-14
+
+
+def test_full_contraction_is_dot_product():
+    """test_ten_ctrct.cc:272-306: contracting every axis gives sum_i a_i b_i."""
+    ix = [Index(((-1,), (0,), (1,)), (2, 3, 2), IN), Index(((0,), (1,)), (2, 2), OUT)]
+    a = BlockSparseTensor.allowed_blocks(ix, (0,)).fill_uniform(8)
+    b = BlockSparseTensor.allowed_blocks([i.inverse() for i in ix], (0,)).fill_uniform(9)
+    p = O.OraclePlan(a, b, [0, 1], [0, 1])
+    assert p.c_rank == 0 and p.c_raw_size == 1
+    assert abs(p.execute()[0] - np.dot(a.data, b.data)) < 1e-13
+
+
+def test_scalar_output_breaks_after_first_match():
+    """data_blk_operations.h:451-491: for a scalar output each A block takes
+    only its first matching B block."""
+    ix = Index(((0,), (1,)), (2, 3), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix, ix], (1,)).fill_uniform(1)
+    b = BlockSparseTensor.allowed_blocks([ix.inverse(), ix.inverse()], (-1,)).fill_uniform(2)
+    p = O.OraclePlan(a, b, [0, 1], [0, 1])
+    t = p.tasks()
+    assert (t["c_data_offset"] == 0).all()
+    assert t["beta"][0] == 0.0 and (t["beta"][1:] == 1.0).all()
+    assert len(set(t["a_blk_idx"])) == len(t)
