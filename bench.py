@@ -1,0 +1,297 @@
+"""Benchmark of the block-sparse Contract hot path on one MI355X.
+
+Workload: the two-site effective-Hamiltonian matvec psi' = L.psi.W.W.R on the
+U(1)xU(1) (N, 2Sz) Hubbard-like tensors, D = 4096 over 85 sectors, Dmpo = 6,
+complex128, uniform(-1,1) entries from a fixed seed (synthetic data).  A
+step is one matvec: the four grouped contractions on cached device plans,
+inputs resident in HBM.  Throughput = algorithmic flops (sum over tasks of
+8*m*k*n, tensor_op_cost.h:384-406) / time.
+
+The JSON line reports the split-BF16 path (``value``) next to the native
+FP64 path, the per-launch roofline of the dominant grouped-GEMM launch
+(HIP events on the launch stream), the error of the step's output against
+the FP64 CPU oracle, and the CPU baseline: the oracle (the reference
+algorithm restated in C, OpenBLAS zgemm per task) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from tensortoolkit_amd import capi, workloads as W  # noqa: E402
+from tensortoolkit_amd.blocksparse import ContractionPlan  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+
+
+class Hip:
+    """hipEvent timing through libamdhip64 (events recorded on the launch stream)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, e0, e1) -> float:
+        assert self.h.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        return float(ms.value)
+
+
+class MatvecRunner:
+    """Device buffers and cached plans of one matvec chain."""
+
+    def __init__(self, mv: W.Matvec, stream):
+        self.mv, self.stream, self.dtype = mv, stream, mv.dtype
+        self.esz = 16 if mv.dtype == "c128" else 8
+        self.bufs = {name: capi.DeviceBuffer.from_numpy(getattr(mv, name).data) for name in ("L", "psi", "W", "R")}
+        t0 = time.perf_counter()
+        self.plans, self.outs, self.out_tensors = [], [], []
+        x = mv.psi
+        for step, axes in enumerate(W.MATVEC_AXES):
+            op = (mv.L, mv.W, mv.W, mv.R)[step]
+            first, second = (op, x) if step == 0 else (x, op)
+            plan = ContractionPlan(first, second, list(axes[0]), list(axes[1]), stream=stream)
+            out = plan.output_tensor(self.dtype)
+            self.plans.append(plan)
+            self.out_tensors.append(out)
+            self.outs.append(capi.DeviceBuffer(max(1, out.raw_size) * self.esz))
+            x = out
+        capi.check(capi.lib().qlt_stream_sync(stream))
+        self.plan_seconds = time.perf_counter() - t0
+        self.flops = [p.flops(self.dtype) for p in self.plans]
+
+    def operands(self, step):
+        names = ("L", "W", "W", "R")
+        if step == 0:
+            return self.bufs["L"].ptr, self.bufs["psi"].ptr
+        return self.outs[step - 1].ptr, self.bufs[names[step]].ptr
+
+    def launch(self, step, mode):
+        a, b = self.operands(step)
+        self.plans[step].execute(self.dtype, mode, a, b, self.outs[step].ptr, stream=self.stream)
+
+    def result(self) -> np.ndarray:
+        out = self.out_tensors[-1]
+        return self.outs[-1].to_numpy(out.np_dtype, out.raw_size)
+
+
+def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int):
+    for p in runner.plans:
+        p.prepare(runner.dtype, mode)
+    for _ in range(warmup):
+        for s in range(4):
+            runner.launch(s, mode)
+    capi.check(capi.lib().qlt_stream_sync(runner.stream))
+    ev = [[(hip.event(), hip.event()) for _ in range(4)] for _ in range(steps)]
+    t_start, t_end = hip.event(), hip.event()
+    capi.check(capi.lib().qlt_device_sync())
+    wall0 = time.perf_counter()
+    hip.record(t_start, runner.stream)
+    for k in range(steps):
+        for s in range(4):
+            hip.record(ev[k][s][0], runner.stream)
+            runner.launch(s, mode)
+            hip.record(ev[k][s][1], runner.stream)
+    hip.record(t_end, runner.stream)
+    capi.check(capi.lib().qlt_stream_sync(runner.stream))
+    capi.check(capi.lib().qlt_device_sync())
+    wall = time.perf_counter() - wall0
+    total_ms = hip.elapsed_ms(t_start, t_end)
+    per_launch = np.array([[hip.elapsed_ms(*ev[k][s]) for s in range(4)] for k in range(steps)])
+    return total_ms, wall, per_launch.mean(axis=0)
+
+
+def time_graph(runner: MatvecRunner, hip: Hip, mode: int, replays: int):
+    lib = capi.lib()
+    capi.check(lib.qlt_graph_begin(runner.stream))
+    for s in range(4):
+        runner.launch(s, mode)
+    g = C.c_void_p()
+    capi.check(lib.qlt_graph_end(runner.stream, C.byref(g)))
+    capi.check(lib.qlt_graph_replay(g, runner.stream))
+    capi.check(lib.qlt_stream_sync(runner.stream))
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, runner.stream)
+    for _ in range(replays):
+        capi.check(lib.qlt_graph_replay(g, runner.stream))
+    hip.record(e1, runner.stream)
+    ms = hip.elapsed_ms(e0, e1)
+    capi.check(lib.qlt_graph_destroy(g))
+    return ms / replays
+
+
+def cpu_baseline(mv: W.Matvec, budget_s: float = 20.0):
+    """The oracle (reference algorithm restated in C, OpenBLAS per task) on a
+    bounded sample: whole matvec steps until about `budget_s` seconds."""
+    from oracle import oracle as O
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores) or cores))
+    info = O.use_openblas(cores)
+    # plans are built once (the reference rebuilds them per call; timed separately)
+    t0 = time.perf_counter()
+    x = mv.psi
+    plans = []
+    for step, axes in enumerate(W.MATVEC_AXES):
+        op = (mv.L, mv.W, mv.W, mv.R)[step]
+        first, second = (op, x) if step == 0 else (x, op)
+        p = O.OraclePlan(first, second, list(axes[0]), list(axes[1]))
+        out = p.output_tensor(mv.dtype)
+        plans.append((p, first is op))
+        x = out
+    plan_s = time.perf_counter() - t0
+    flops_step = sum((8.0 if mv.dtype == "c128" else 2.0) * float((p.tasks()["m"] * p.tasks()["k"] * p.tasks()["n"]).sum())
+                     for p, _ in plans)
+    done, t_exec, result = 0, 0.0, None
+    while True:
+        t0 = time.perf_counter()
+        x = mv.psi
+        for step, (p, _) in enumerate(plans):
+            if step > 0:
+                p.a.data = x
+            out = p.execute()
+            x = out
+        t_exec += time.perf_counter() - t0
+        done += 1
+        result = x
+        if t_exec >= budget_s or done >= 3:
+            break
+    return {
+        "value": flops_step * done / t_exec / 1e9,
+        "unit": "GFLOP/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{done} full matvec step(s), execute only ({t_exec:.1f} s); plan build {plan_s:.2f} s "
+                  f"once; OpenBLAS zgemm per task, {info['threads']} threads ({info['library']})",
+    }, result
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph-replays", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the contraction would shard by output block, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    capi.init(0)
+    stream = C.c_void_p()
+    capi.check(capi.lib().qlt_stream_create(C.byref(stream)))
+    hip = Hip()
+    t0 = time.perf_counter()
+    mv = {"u1u1_D4096": W.config3, "u1_D2048": W.config1, "u1_lognormal": W.config2}[args.config]()
+    gen_s = time.perf_counter() - t0
+    runner = MatvecRunner(mv, stream)
+    flops_step = float(sum(runner.flops))
+    print(f"[bench] {mv.name}: tensors built in {gen_s:.1f}s, plans in {runner.plan_seconds:.2f}s, "
+          f"tasks {[int(p.info.n_tasks) for p in runner.plans]}, GFLOP/step {flops_step / 1e9:.2f}",
+          file=sys.stderr, flush=True)
+
+    results = {}
+    modes = [("native", capi.QLT_MODE_NATIVE), ("split_bf16", capi.QLT_MODE_SPLIT_BF16)]
+    outputs = {}
+    for name, mode in modes:
+        try:
+            total_ms, wall, per_launch = time_mode(runner, hip, mode, args.steps, args.warmup)
+        except capi.QltError as e:
+            print(f"[bench] mode {name} unavailable: {e}", file=sys.stderr, flush=True)
+            continue
+        outputs[name] = runner.result()
+        graph_ms = time_graph(runner, hip, mode, args.graph_replays) if args.graph_replays else None
+        results[name] = dict(total_ms=total_ms, wall=wall, per_launch_ms=per_launch, graph_ms=graph_ms)
+        print(f"[bench] {name}: {total_ms / args.steps:.3f} ms/step, per launch {np.round(per_launch, 3).tolist()}"
+              f", graph replay {graph_ms:.3f} ms/step" if graph_ms else "", file=sys.stderr, flush=True)
+
+    # correctness of the step output against the FP64 CPU oracle
+    errors = {}
+    cpu = None
+    ref = None
+    if not args.no_cpu_baseline:
+        cpu, ref = cpu_baseline(mv)
+        out_t = runner.out_tensors[-1]
+        for name, got in outputs.items():
+            worst = 0.0
+            for i in range(out_t.n_blks):
+                sz = int(np.prod(out_t.dims[i]))
+                o = out_t.offsets[i]
+                den = np.linalg.norm(ref[o:o + sz])
+                if den > 0:
+                    worst = max(worst, float(np.linalg.norm(got[o:o + sz] - ref[o:o + sz]) / den))
+            errors[name] = worst
+
+    primary = "split_bf16" if "split_bf16" in results else "native"
+    r = results[primary]
+    ms_step = r["total_ms"] / args.steps
+    value = flops_step / (ms_step * 1e-3) / 1e9
+    dom = int(np.argmax(runner.flops))
+    dom_ms = float(r["per_launch_ms"][dom])
+    achieved_tf = runner.flops[dom] / (dom_ms * 1e-3) / 1e12
+    native_gf = None
+    if "native" in results:
+        native_gf = flops_step / (results["native"]["total_ms"] / args.steps * 1e-3) / 1e9
+    line = {
+        "metric": "block-sparse Contract GFLOP/s, split-BF16 grouped vs FP64, at <=1e-12 rel err",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if mv.dtype == "c128" else "f64",
+        "data": "synthetic",
+        "config": {"workload": mv.name, "path": primary, "contractions_per_step": 4,
+                   "gflop_per_step": round(flops_step / 1e9, 3),
+                   "tasks": [int(p.info.n_tasks) for p in runner.plans]},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 3),
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "kernel": f"grouped GEMM launch of contraction {dom} ({primary})"},
+        "native_gflops": round(native_gf, 2) if native_gf else None,
+        "split_vs_native": round(value / native_gf, 3) if (native_gf and primary != "native") else None,
+        "max_rel_err_vs_oracle": {k: float(f"{v:.3e}") for k, v in errors.items()},
+        "graph_replay_ms_per_step": {k: round(v["graph_ms"], 4) for k, v in results.items() if v["graph_ms"]},
+        "plan_build_s": round(runner.plan_seconds, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

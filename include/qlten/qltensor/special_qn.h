@@ -1,0 +1,90 @@
+// Quantum-number types of the contraction path: U(1) and U(1)xU(1).
+// API as the reference's special_qn::U1QN (qltensor/special_qn/u1qn.h:27-169)
+// and special_qn::U1U1QN (qltensor/special_qn/u1u1qn.h:27-189).  Equality is by
+// hash, as in the reference; the hash maps are kept identical so that Index
+// equality (hash based) agrees with the reference's.
+#ifndef QLTEN_AMD_QLTENSOR_SPECIAL_QN_H
+#define QLTEN_AMD_QLTENSOR_SPECIAL_QN_H
+
+#include <cstddef>
+#include <iostream>
+#include <string>
+
+namespace qlten {
+namespace special_qn {
+
+class U1QN {
+ public:
+  U1QN() : U1QN(0) {}
+  U1QN(const int val) : val_(val), hash_(CalcHash(val)) {}  // NOLINT: implicit like the reference
+  U1QN(const std::string &, const int val) : U1QN(val) {}
+
+  U1QN operator-() const { return U1QN(-val_); }
+  U1QN &operator+=(const U1QN &rhs) {
+    val_ += rhs.val_;
+    hash_ = CalcHash(val_);
+    return *this;
+  }
+  U1QN operator+(const U1QN &rhs) const { return U1QN(val_ + rhs.val_); }
+  U1QN operator-(const U1QN &rhs) const { return U1QN(val_ - rhs.val_); }
+  bool operator==(const U1QN &rhs) const { return hash_ == rhs.hash_; }
+  bool operator!=(const U1QN &rhs) const { return !(*this == rhs); }
+
+  size_t dim() const { return 1; }
+  size_t Hash() const { return hash_; }
+  int GetVal() const { return val_; }
+  static U1QN Zero() { return U1QN(0); }
+
+ private:
+  // the value, rotated left by 31 bits (u1qn.h:139-152)
+  static size_t CalcHash(int v) {
+    const size_t x = static_cast<size_t>(v);
+    return (x << 31) | (x >> 33);
+  }
+  int val_;
+  size_t hash_;
+};
+
+class U1U1QN {
+ public:
+  U1U1QN() : U1U1QN(0, 0) {}
+  U1U1QN(const int v1, const int v2) : vals_{v1, v2}, hash_(CalcHash(v1, v2)) {}
+  U1U1QN(const std::string &, const int v1, const std::string &, const int v2) : U1U1QN(v1, v2) {}
+
+  U1U1QN operator-() const { return U1U1QN(-vals_[0], -vals_[1]); }
+  U1U1QN &operator+=(const U1U1QN &rhs) {
+    vals_[0] += rhs.vals_[0];
+    vals_[1] += rhs.vals_[1];
+    hash_ = CalcHash(vals_[0], vals_[1]);
+    return *this;
+  }
+  U1U1QN operator+(const U1U1QN &rhs) const { return U1U1QN(vals_[0] + rhs.vals_[0], vals_[1] + rhs.vals_[1]); }
+  U1U1QN operator-(const U1U1QN &rhs) const { return U1U1QN(vals_[0] - rhs.vals_[0], vals_[1] - rhs.vals_[1]); }
+  bool operator==(const U1U1QN &rhs) const { return hash_ == rhs.hash_; }
+  bool operator!=(const U1U1QN &rhs) const { return !(*this == rhs); }
+
+  size_t dim() const { return 1; }
+  size_t Hash() const { return hash_; }
+  int GetVal(size_t i) const { return vals_[i]; }
+  static U1U1QN Zero() { return U1U1QN(0, 0); }
+
+ private:
+  // both values offset by 2^30 and packed as v1 + 2^31 v2, rotated left by
+  // 10 bits (u1u1qn.h:151-173)
+  static size_t CalcHash(int v1, int v2) {
+    const size_t seg = size_t(1) << 30;
+    const size_t h = (static_cast<size_t>(v1) + seg) + (static_cast<size_t>(v2) + seg) * (2 * seg);
+    return (h << 10) | (h >> 54);
+  }
+  int vals_[2];
+  size_t hash_;
+};
+
+}  // namespace special_qn
+
+using special_qn::U1QN;
+using special_qn::U1U1QN;
+
+}  // namespace qlten
+
+#endif

@@ -169,6 +169,15 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
                                 const double *alpha, const double *beta,
                                 void *stream);
 
+/* Split-mode dispatch threshold: a plan whose flop-weighted mean contracted
+ * extent k is below `min_mean_k` runs on the native kernel even in
+ * QLT_MODE_SPLIT_BF16 (such contractions are bound by memory traffic, where
+ * slicing only adds traffic).  Default 48; 0 forces the split kernel.  A
+ * runtime knob in the spirit of the reference's SetCpuNumericsThreads
+ * (framework/hp_numeric/omp_set.h:67-83); it has no reference counterpart. */
+qlt_status qlt_set_split_min_mean_k(double min_mean_k);
+double qlt_get_split_min_mean_k(void);
+
 /* ---- graph capture of a repeated contraction chain (Lanczos / DMRG sweep) ---- */
 qlt_status qlt_graph_begin(void *stream);
 qlt_status qlt_graph_end(void *stream, qlt_graph **graph);
@@ -176,12 +185,13 @@ qlt_status qlt_graph_replay(qlt_graph *graph, void *stream);
 qlt_status qlt_graph_destroy(qlt_graph *graph);
 
 /* ---- raw-data elementwise primitives --------------------------------------
- * Seeded fill used for inputs: element i (real: value i; complex: re = value
- * 2i, im = value 2i+1) is 2u-1 with u = (SplitMix64(seed + (i+1)*0x9E3779B97F4A7C15) >> 11) * 2^-53.
- * It replaces the reference's curand/mt19937 Random (raw_data_operations.h:141-166),
- * whose stream is implementation-defined. */
+ * Seeded fill: double i of x (complex data: re = double 2j, im = double 2j+1)
+ * is lo + (hi - lo) * u with u = (SplitMix64(seed + (first_index+i+1) * 0x9E3779B97F4A7C15) >> 11) * 2^-53.
+ * (lo, hi) = (0, 1) gives the reference's Random range [0, 1); (-1, 1) gives
+ * 2u - 1.  It replaces the curand / mt19937 fill of Random
+ * (raw_data_operations.h:141-166), whose stream is implementation-defined. */
 qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t first_index,
-                            void *stream);
+                            double lo, double hi, void *stream);
 /* y = a*x + b*y over n elements (LinearCombine building block, ten_linear_combine.h:186). */
 qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x,
                      const double *b, void *y, void *stream);

@@ -585,7 +585,7 @@ int ora_execute(const ora_plan *p, int cplx, const double *A, const double *B, d
 }
 
 /* SplitMix64-based uniform(-1,1) fill, the input generator shared with the
- * device (qlt_fill_uniform): value i = 2*((z >> 11) * 2^-53) - 1. */
+ * device (qlt_fill_uniform with lo = -1, hi = 1): value i = -1 + 2 * ((z >> 11) * 2^-53). */
 void ora_fill_uniform(double *x, int64_t n, uint64_t seed, int64_t first) {
 #pragma omp parallel for num_threads(g_threads) schedule(static) if (n > 1000000)
   for (int64_t i = 0; i < n; ++i) {
@@ -593,6 +593,6 @@ void ora_fill_uniform(double *x, int64_t n, uint64_t seed, int64_t first) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    x[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+    x[i] = -1.0 + 2.0 * ((double)(z >> 11) * 0x1.0p-53);
   }
 }

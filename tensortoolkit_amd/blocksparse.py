@@ -64,7 +64,7 @@ def splitmix_uniform(n: int, seed: int, first: int = 0) -> np.ndarray:
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         z = z ^ (z >> np.uint64(31))
-    return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
+    return -1.0 + 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53)
 
 
 class BlockSparseTensor:

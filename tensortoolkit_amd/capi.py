@@ -98,11 +98,14 @@ _SIGS = {
     "qlt_plan_prepare": ([C.c_void_p, C.c_int32, C.c_int32], C.c_int32),
     "qlt_contract_execute": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                               _f64p, _f64p, C.c_void_p], C.c_int32),
+    "qlt_set_split_min_mean_k": ([C.c_double], C.c_int32),
+    "qlt_get_split_min_mean_k": ([], C.c_double),
     "qlt_graph_begin": ([C.c_void_p], C.c_int32),
     "qlt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int32),
     "qlt_graph_replay": ([C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_graph_destroy": ([C.c_void_p], C.c_int32),
-    "qlt_fill_uniform": ([C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, C.c_void_p], C.c_int32),
+    "qlt_fill_uniform": ([C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p],
+                         C.c_int32),
     "qlt_axpby": ([C.c_int32, C.c_int64, _f64p, C.c_void_p, _f64p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_dot": ([C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, _f64p, C.c_void_p], C.c_int32),
     "qlt_real_to_complex": ([C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),

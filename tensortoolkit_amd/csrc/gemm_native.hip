@@ -201,27 +201,182 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       }
 }
 
+// Row-streaming kernel for narrow outputs (n <= kRowsMaxN): one thread per
+// output row, 256 rows per workgroup.  Each k-chunk of A' (256 rows x 16) is
+// staged in LDS with loads contiguous along A's innermost memory axis (rows
+// or k), the matching 16 x n chunk of B' is staged once per workgroup, and
+// every thread accumulates its n outputs with FP64 FMAs.  The work is
+// bound by reading A and writing C once.
+constexpr int RK = 16;            // k per stage
+constexpr int RLD = kRowsPerTile + 1;
+
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_rows_f64(
+    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
+    const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
+    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
+    double alpha_i, double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  __shared__ double As[P][RK][RLD];
+  __shared__ double Bs[P][RK][kRowsMaxN];
+  __shared__ int32_t s_arow[kRowsPerTile];
+  __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[kRowsMaxN];
+
+  const Tile tile = tiles[blockIdx.x];
+  const Seg seg = segs[tile.seg];
+  const int tid = threadIdx.x;
+  const int r0 = tile.tm * kRowsPerTile;
+  const int row = r0 + tid;
+  const int n = seg.n;
+
+  double acc[kRowsMaxN], acc_i[kRowsMaxN];
+#pragma unroll
+  for (int j = 0; j < kRowsMaxN; ++j) {
+    acc[j] = 0.0;
+    acc_i[j] = 0.0;
+  }
+
+  for (int t = seg.t0; t < seg.t1; ++t) {
+    const ExecTask et = exec[t];
+    const BlkGather &gA = ga[et.a_blk];
+    const BlkGather &gB = gb[et.b_blk];
+    __syncthreads();
+    s_arow[tid] = (row < seg.m) ? split_offset(row, gA.rdim, gA.rstr, L.a_nr) : -1;
+    if (tid < kRowsMaxN) s_bcol[tid] = (tid < n) ? split_offset(tid, gB.cdim, gB.cstr, L.b_nc) : -1;
+    for (int kk = 0; kk < et.k; kk += RK) {
+      if (tid >= 128 && tid < 128 + RK) {
+        const int p = kk + tid - 128;
+        s_acol[tid - 128] = (p < et.k) ? split_offset(p, gA.cdim, gA.cstr, L.a_nc) : -1;
+      } else if (tid >= 192 && tid < 192 + RK) {
+        const int p = kk + tid - 192;
+        s_brow[tid - 192] = (p < et.k) ? split_offset(p, gB.rdim, gB.rstr, L.b_nr) : -1;
+      }
+      __syncthreads();
+      // A' chunk: 256 rows x RK
+#pragma unroll 4
+      for (int q = 0; q < RK; ++q) {
+        int r, p;
+        if (L.a_kfast) {
+          p = tid & (RK - 1);
+          r = (tid >> 4) + 16 * q;
+        } else {
+          r = tid;
+          p = q;
+        }
+        const int32_t ro = s_arow[r], co = s_acol[p];
+        const bool ok = (ro >= 0) && (co >= 0);
+        const int64_t e = et.a_off + ro + co;
+        if (CPLX) {
+          const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
+          As[0][p][r] = v.x;
+          As[P - 1][p][r] = v.y;
+        } else {
+          As[0][p][r] = ok ? A[e] : 0.0;
+        }
+      }
+      // B' chunk: RK x n
+      {
+        const int p = tid / kRowsMaxN, j = tid % kRowsMaxN;
+        const int32_t ro = s_brow[p], co = s_bcol[j];
+        const bool ok = (ro >= 0) && (co >= 0);
+        const int64_t e = et.b_off + ro + co;
+        if (CPLX) {
+          const double2 v = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
+          Bs[0][p][j] = v.x;
+          Bs[P - 1][p][j] = v.y;
+        } else {
+          Bs[0][p][j] = ok ? B[e] : 0.0;
+        }
+      }
+      __syncthreads();
+      const int kc = min(RK, et.k - kk);
+      for (int p = 0; p < kc; ++p) {
+        const double a = As[0][p][tid];
+        const double a_im = As[P - 1][p][tid];
+#pragma unroll
+        for (int j = 0; j < kRowsMaxN; ++j) {
+          if (j < n) {
+            const double b = Bs[0][p][j];
+            if (CPLX) {
+              const double b_im = Bs[P - 1][p][j];
+              acc[j] = fma(a, b, acc[j]);
+              acc[j] = fma(-a_im, b_im, acc[j]);
+              acc_i[j] = fma(a, b_im, acc_i[j]);
+              acc_i[j] = fma(a_im, b, acc_i[j]);
+            } else {
+              acc[j] = fma(a, b, acc[j]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (row >= seg.m) return;
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  const int64_t base = seg.c_off + static_cast<int64_t>(row) * n;
+#pragma unroll
+  for (int j = 0; j < kRowsMaxN; ++j) {
+    if (j < n) {
+      if (CPLX) {
+        double yr = alpha_r * acc[j] - alpha_i * acc_i[j], yi = alpha_r * acc_i[j] + alpha_i * acc[j];
+        double2 *cp = reinterpret_cast<double2 *>(C) + base + j;
+        if (use_beta) {
+          const double2 o = *cp;
+          yr += beta_r * o.x - beta_i * o.y;
+          yi += beta_r * o.y + beta_i * o.x;
+        }
+        *cp = double2{yr, yi};
+      } else {
+        double y = alpha_r * acc[j];
+        if (use_beta) y += beta_r * C[base + j];
+        C[base + j] = y;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 namespace qlt {
 
-// Launch the native grouped kernel for a plan (declared in gemm_split.hip's
-// dispatcher as well).
-qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
-                         const double *alpha, const double *beta, hipStream_t st) {
+// Native grouped execution of a plan: MFMA tiles for the first tile class,
+// the row-streaming kernel for the second.  `mfma_class` = false skips the
+// MFMA class (the split path runs it on its own kernel).
+qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class) {
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
-  const dim3 grid(static_cast<unsigned>(plan->n_tiles)), block(256);
-  if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_grouped_gemm_f64<false>, grid, block, 0, st, plan->d_tiles, plan->d_segs,
-                       plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, static_cast<const double *>(a),
-                       static_cast<const double *>(b), static_cast<double *>(c), ar, ai, br, bi);
-  else
-    hipLaunchKernelGGL(k_grouped_gemm_f64<true>, grid, block, 0, st, plan->d_tiles, plan->d_segs,
-                       plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, static_cast<const double *>(a),
-                       static_cast<const double *>(b), static_cast<double *>(c), ar, ai, br, bi);
+  const dim3 block(256);
+  const int64_t n0 = plan->cls_tiles[kTileMfma], n1 = plan->cls_tiles[kTileRows];
+  const auto *A = static_cast<const double *>(a);
+  const auto *B = static_cast<const double *>(b);
+  auto *Cp = static_cast<double *>(c);
+  if (mfma_class && n0 > 0) {
+    if (dtype == QLT_F64)
+      hipLaunchKernelGGL(k_grouped_gemm_f64<false>, dim3(static_cast<unsigned>(n0)), block, 0, st, plan->d_tiles,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+    else
+      hipLaunchKernelGGL(k_grouped_gemm_f64<true>, dim3(static_cast<unsigned>(n0)), block, 0, st, plan->d_tiles,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+  }
+  if (n1 > 0) {
+    const Tile *t1 = plan->d_tiles + n0;
+    if (dtype == QLT_F64)
+      hipLaunchKernelGGL(k_rows_f64<false>, dim3(static_cast<unsigned>(n1)), block, 0, st, t1, plan->d_segs,
+                         plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+    else
+      hipLaunchKernelGGL(k_rows_f64<true>, dim3(static_cast<unsigned>(n1)), block, 0, st, t1, plan->d_segs,
+                         plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+  }
   QLT_HIP(hipGetLastError());
   return QLT_OK;
+}
+
+qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+                         const double *alpha, const double *beta, hipStream_t st) {
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, true);
 }
 
 }  // namespace qlt

@@ -1,18 +1,586 @@
-// Split-precision (BF16-sliced, FP64-emulating) grouped GEMM.
+// Split-precision grouped block GEMM: FP64 (and complex128) emulated on the
+// BF16 MFMA units.
+//
+// Same contract as the native kernel (it replaces the per-task MatMultiply
+// loop of CtrctTwoBSDTAndAssignIn, global_operations.h:895-992); only the
+// arithmetic differs.  Two launches per execute:
+//
+// 1. pack: every block the plan touches is read once, through the same gather
+//    descriptors as the native kernel (so the transposes stay folded), scaled
+//    per row of A' and per column of B' by a power of two 2^e with
+//    max|x| / 2^e in [1/2, 1], and split by round-to-nearest into S = 5 slices
+//        y = x / 2^e,  c1 = rint(y * 2^8),  r1 = y - c1 2^-8,
+//        c_s = rint(r_{s-1} * 2^(8 + 9(s-1))),  |c_s| <= 256,
+//    so y = 2^-8 * sum_s c_s 2^(-9(s-1)) + O(2^-53).  Slice s is stored as the
+//    BF16 number c_s 2^(-9(s-1)) (integers |c| <= 256 have at most 8
+//    significant bits, so every slice is exact in BF16).
+// 2. GEMM: C' = sum over slice pairs with sa + sb <= S + 1 (15 products) of
+//    A_sa B_sb on v_mfma_f32_32x32x16_bf16.  Products of level L = sa + sb
+//    carry the weight 2^(-9(L-2)) in their BF16 values.  They go to four FP32
+//    accumulators {L=2}, {L=3}, {L=4}, {L=5,6}; each is flushed to FP64 after
+//    at most KCH k-steps, short enough that the first three stay exact
+//    (|partial sum| < 2^24 ulps of its level), and the flush sum
+//    g2 + g3 + g4 + g56 is exact in FP64.  The flushed value is scaled by
+//    2^(e_row + e_col - 16) with ldexp and added to the FP64 accumulator.
+//    Truncating at level 6 leaves a relative error of order 2^-45.
+// Complex128 is the 4-real-product form Cr = Ar Br - Ai Bi, Ci = Ar Bi + Ai Br
+// on the same slices, with one exponent per row/column over both parts.
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "qlt_internal.h"
+
+using namespace qlt;
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+constexpr int S = 5;          // slices per operand
+constexpr int KC = 16;        // k per LDS stage (one MFMA k-step)
+constexpr int TM = 64;        // tile rows
+constexpr int TN = 64;        // tile cols
+constexpr int RPAD = 64;      // packed rows are padded to a multiple of this
+constexpr int LDK = KC + 8;   // padded LDS row (bf16)
+
+// Packed layout of one block (rows R_pad = round_up(R, 64), k_pad = round_up(K, 16)):
+//   plane element (part, s, r, p) at
+//   (((part * S + s) * (k_pad / KC) + p / KC) * R_pad + r) * KC + p % KC
+struct PackMeta {
+  int64_t plane_off;  // bf16 elements into the plane workspace
+  int64_t exp_off;    // int32 elements into the exponent workspace
+  int32_t rows;       // R (actual)
+  int32_t r_pad;
+  int32_t cols;       // K (actual)
+  int32_t k_pad;
+};
+
+struct PackItem {
+  int32_t side;       // 0: A block (rows = saved, cols = contracted); 1: B block transposed
+  int32_t blk;        // block position
+  int32_t r0;         // first row of this 64-row group
+  int32_t pad;
+};
+
+__device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, const int32_t *strides,
+                                                int n) {
+  int32_t off = 0;
+  for (int j = n - 1; j >= 0; --j) {
+    const int32_t d = dims[j];
+    const int32_t q = x / d;
+    off += (x - q * d) * strides[j];
+    x = q;
+  }
+  return off;
+}
+
+__device__ __forceinline__ uint16_t bf16_bits_exact(double v) {
+  // v is c * 2^-m with |c| <= 256 an integer: exactly representable in BF16,
+  // so truncating the FP32 bit pattern is exact.
+  const float f = static_cast<float>(v);
+  return static_cast<uint16_t>(__float_as_uint(f) >> 16);
+}
+
+// Pack kernel: one workgroup of 256 threads per (side, block, 64-row group).
+// Pass 1 finds the max |x| of each row over all k; pass 2 splits and writes.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items,
+                                              const PackMeta *__restrict__ meta_a,
+                                              const PackMeta *__restrict__ meta_b,
+                                              const BlkGather *__restrict__ ga,
+                                              const BlkGather *__restrict__ gb,
+                                              const int64_t *__restrict__ a_offs,
+                                              const int64_t *__restrict__ b_offs, const GatherLayout L,
+                                              const double *__restrict__ A, const double *__restrict__ B,
+                                              uint16_t *__restrict__ planes, int32_t *__restrict__ exps) {
+  constexpr int KP = CPLX ? 32 : 64;   // k per pass-2 chunk
+  constexpr int PARTS = CPLX ? 2 : 1;
+  __shared__ int32_t s_roff[64];
+  __shared__ int32_t s_coff[64];
+  __shared__ unsigned long long s_max[64];
+  __shared__ double s_tile[PARTS][64][KP + 1];
+
+  const PackItem it = items[blockIdx.x];
+  const bool is_a = (it.side == 0);
+  const PackMeta pm = is_a ? meta_a[it.blk] : meta_b[it.blk];
+  const BlkGather &g = is_a ? ga[it.blk] : gb[it.blk];
+  const int64_t base = is_a ? a_offs[it.blk] : b_offs[it.blk];
+  const double *src = is_a ? A : B;
+  // A': rows = saved (rdim), cols = contracted (cdim).  B'^T: rows = saved
+  // (cdim of B's descriptor), cols = contracted (rdim of B's descriptor).
+  const int32_t *rdim = is_a ? g.rdim : g.cdim;
+  const int32_t *rstr = is_a ? g.rstr : g.cstr;
+  const int32_t *cdim = is_a ? g.cdim : g.rdim;
+  const int32_t *cstr = is_a ? g.cstr : g.rstr;
+  const int nr = is_a ? L.a_nr : L.b_nc;
+  const int nc = is_a ? L.a_nc : L.b_nr;
+  // whether consecutive k are consecutive in memory
+  const bool kfast = is_a ? (L.a_kfast != 0) : (L.b_nfast == 0);
+  const int tid = threadIdx.x;
+
+  if (tid < 64) {
+    const int r = it.r0 + tid;
+    s_roff[tid] = (r < pm.rows) ? split_offset(r, rdim, rstr, nr) : -1;
+    s_max[tid] = 0ull;
+  }
+  __syncthreads();
+
+  // ---- pass 1: row maxima ----
+  for (int p0 = 0; p0 < pm.cols; p0 += 64) {
+    if (tid < 64) {
+      const int p = p0 + tid;
+      s_coff[tid] = (p < pm.cols) ? split_offset(p, cdim, cstr, nc) : -1;
+    }
+    __syncthreads();
+    for (int q = 0; q < 16; ++q) {
+      int r, p;
+      if (kfast) {
+        p = tid & 63;
+        r = (tid >> 6) + 4 * q;
+      } else {
+        r = tid & 63;
+        p = (tid >> 6) + 4 * q;
+      }
+      const int32_t ro = s_roff[r], co = s_coff[p];
+      if (ro >= 0 && co >= 0) {
+        const int64_t e = base + ro + co;
+        double m;
+        if (CPLX) {
+          const double2 v = reinterpret_cast<const double2 *>(src)[e];
+          m = fmax(fabs(v.x), fabs(v.y));
+        } else {
+          m = fabs(src[e]);
+        }
+        atomicMax(&s_max[r], static_cast<unsigned long long>(__double_as_longlong(m)));
+      }
+    }
+    __syncthreads();
+  }
+
+  // exponent of each row: max / 2^e in [1/2, 1]
+  __shared__ int32_t s_exp[64];
+  if (tid < 64) {
+    const double m = __longlong_as_double(static_cast<long long>(s_max[tid]));
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);
+    s_exp[tid] = e;
+    const int r = it.r0 + tid;
+    if (r < pm.r_pad) exps[pm.exp_off + r] = e;
+  }
+  __syncthreads();
+
+  // ---- pass 2: split and write (including the zero padding) ----
+  const int nkc = pm.k_pad / KC;
+  for (int p0 = 0; p0 < pm.k_pad; p0 += KP) {
+    if (tid < KP) {
+      const int p = p0 + tid;
+      s_coff[tid] = (p < pm.cols) ? split_offset(p, cdim, cstr, nc) : -1;
+    }
+    __syncthreads();
+    for (int q = 0; q < (64 * KP) / 256; ++q) {
+      int r, p;
+      if (kfast) {
+        p = tid % KP;
+        r = tid / KP + (256 / KP) * q;
+      } else {
+        r = tid & 63;
+        p = (tid >> 6) + 4 * q;
+      }
+      const int32_t ro = s_roff[r], co = s_coff[p];
+      const bool ok = (ro >= 0) && (co >= 0);
+      const int64_t e = base + ro + co;
+      if (CPLX) {
+        const double2 v = ok ? reinterpret_cast<const double2 *>(src)[e] : double2{0.0, 0.0};
+        s_tile[0][r][p] = v.x;
+        s_tile[PARTS - 1][r][p] = v.y;
+      } else {
+        s_tile[0][r][p] = ok ? src[e] : 0.0;
+      }
+    }
+    __syncthreads();
+    // thread -> (row, run of KP/4 consecutive k)
+    constexpr int RUN = KP / 4;
+    const int r = tid >> 2;
+    const int pr = (tid & 3) * RUN;
+    const int grow = it.r0 + r;
+    if (grow < pm.r_pad) {
+      const double inv = ldexp(1.0, -s_exp[r]);
+#pragma unroll
+      for (int part = 0; part < PARTS; ++part) {
+        uint16_t out[S][RUN];
+#pragma unroll
+        for (int u = 0; u < RUN; ++u) {
+          double y = s_tile[part][r][pr + u] * inv;   // exact (power of two)
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const double scale = ldexp(1.0, 8 + 9 * s);
+            const double c = rint(y * scale);           // |c| <= 256
+            y -= c / scale;                             // exact
+            out[s][u] = bf16_bits_exact(c * ldexp(1.0, -9 * s));
+          }
+        }
+        const int p = p0 + pr;
+        if (p < pm.k_pad) {
+          // RUN consecutive k never cross a KC boundary (RUN divides KC or equals it)
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const int64_t idx =
+                pm.plane_off +
+                ((static_cast<int64_t>(part * S + s) * nkc + p / KC) * pm.r_pad + grow) * KC + (p % KC);
+            uint16_t *dst = planes + idx;
+#pragma unroll
+            for (int u = 0; u < RUN; u += 8) {
+              v4u w;
+              w[0] = static_cast<uint32_t>(out[s][u + 0]) | (static_cast<uint32_t>(out[s][u + 1]) << 16);
+              w[1] = static_cast<uint32_t>(out[s][u + 2]) | (static_cast<uint32_t>(out[s][u + 3]) << 16);
+              w[2] = static_cast<uint32_t>(out[s][u + 4]) | (static_cast<uint32_t>(out[s][u + 5]) << 16);
+              w[3] = static_cast<uint32_t>(out[s][u + 6]) | (static_cast<uint32_t>(out[s][u + 7]) << 16);
+              *reinterpret_cast<v4u *>(dst + u) = w;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ v8bf neg8(v8bf x) {
+  v4u u = __builtin_bit_cast(v4u, x);
+  u ^= 0x80008000u;
+  return __builtin_bit_cast(v8bf, u);
+}
+
+// Level group of a slice pair (sa, sb are 0-based, level = sa + sb + 2).
+__device__ __forceinline__ constexpr int group_of(int sa, int sb) {
+  return (sa + sb >= 3) ? 3 : (sa + sb);
+}
+
+// Split GEMM: one 256-thread workgroup per 64 x 64 output tile, 2 x 2 waves of
+// 32 x 32, one 32x32x16 MFMA block per slice product.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_split_gemm(
+    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
+    const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
+    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int PARTS = CPLX ? 2 : 1;
+  constexpr int KCH = CPLX ? 32 : 64;   // k between flushes (keeps levels 2..4 exact)
+  constexpr int NOUT = CPLX ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) uint16_t As[PARTS][S][TM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[PARTS][S][TN][LDK];
+  __shared__ int32_t s_ea[TM];
+
+  const Tile tile = tiles[blockIdx.x];
+  const Seg seg = segs[tile.seg];
+  const int i0 = tile.tm * TM, j0 = tile.tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int lrow = lane & 31, lk = (lane >> 5) * 8;
+
+  double acc64[NOUT][16];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc64[o][r] = 0.0;
+  v16f g[NOUT][4];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
+
+  for (int t = seg.t0; t < seg.t1; ++t) {
+    const ExecTask et = exec[t];
+    const PackMeta ma = meta_a[et.a_blk];
+    const PackMeta mb = meta_b[et.b_blk];
+    __syncthreads();
+    if (tid < TM) s_ea[tid] = exps[ma.exp_off + i0 + tid];
+    const int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+    const int nkc = ma.k_pad / KC;
+    int since_flush = 0;
+    for (int kc = 0; kc < nkc; ++kc) {
+      // ---- stage the KC-chunk of every slice plane of A and B in LDS ----
+      // per (part, s): 64 rows x 16 bf16 = 2 KB contiguous = 128 x 16 B
+      constexpr int CHUNKS = PARTS * S * 128;
+      for (int c = tid; c < CHUNKS; c += 256) {
+        const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+        const int64_t ia = ma.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * ma.r_pad + i0 + rr) * KC + half * 8;
+        const int64_t ib = mb.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * mb.r_pad + j0 + rr) * KC + half * 8;
+        *reinterpret_cast<v4u *>(&As[ps / S][ps % S][rr][half * 8]) = *reinterpret_cast<const v4u *>(planes + ia);
+        *reinterpret_cast<v4u *>(&Bs[ps / S][ps % S][rr][half * 8]) = *reinterpret_cast<const v4u *>(planes + ib);
+      }
+      __syncthreads();
+      // ---- 15 slice products (x4 real products for complex) ----
+      v8bf bfr[PARTS][S];
+#pragma unroll
+      for (int part = 0; part < PARTS; ++part)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          bfr[part][s] = *reinterpret_cast<const v8bf *>(&Bs[part][s][wn + lrow][lk]);
+#pragma unroll
+      for (int sa = 0; sa < S; ++sa) {
+        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[0][sa][wm + lrow][lk]);
+        v8bf ai, nai;
+        if (CPLX) {
+          ai = *reinterpret_cast<const v8bf *>(&As[PARTS - 1][sa][wm + lrow][lk]);
+          nai = neg8(ai);
+        }
+#pragma unroll
+        for (int sb = 0; sb < S - sa; ++sb) {
+          const int q = group_of(sa, sb);
+          g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[0][sb], g[0][q], 0, 0, 0);
+          if (CPLX) {
+            g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nai, bfr[PARTS - 1][sb], g[0][q], 0, 0, 0);
+            g[NOUT - 1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[PARTS - 1][sb], g[NOUT - 1][q], 0, 0, 0);
+            g[NOUT - 1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, bfr[0][sb], g[NOUT - 1][q], 0, 0, 0);
+          }
+        }
+      }
+      since_flush += KC;
+      // ---- flush to FP64 ----
+      if (since_flush >= KCH || kc == nkc - 1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const int e = s_ea[row] + eb - 16;
+#pragma unroll
+          for (int o = 0; o < NOUT; ++o) {
+            const double sum = static_cast<double>(g[o][0][r]) + static_cast<double>(g[o][1][r]) +
+                               static_cast<double>(g[o][2][r]) + static_cast<double>(g[o][3][r]);
+            acc64[o][r] += ldexp(sum, e);
+          }
+        }
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
+        since_flush = 0;
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- epilogue ----
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  const int col = j0 + wn + lrow;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (row < seg.m && col < seg.n) {
+      const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
+      if (CPLX) {
+        const double xr = acc64[0][r], xi = acc64[NOUT - 1][r];
+        double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+        double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+        if (use_beta) {
+          const double2 o = *cp;
+          yr += beta_r * o.x - beta_i * o.y;
+          yi += beta_r * o.y + beta_i * o.x;
+        }
+        *cp = double2{yr, yi};
+      } else {
+        double y = alpha_r * acc64[0][r];
+        if (use_beta) y += beta_r * C[idx];
+        C[idx] = y;
+      }
+    }
+  }
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Host-side layout of the split workspace of a plan.
+struct SplitLayout {
+  std::vector<PackMeta> meta_a, meta_b;
+  std::vector<PackItem> items;
+  int64_t plane_elems = 0;
+  int64_t exp_elems = 0;
+  size_t off_meta_a = 0, off_meta_b = 0, off_items = 0, off_aoffs = 0, off_boffs = 0, off_exps = 0,
+         off_planes = 0, total = 0;
+};
+
+SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
+  SplitLayout L;
+  const int parts = (dtype == QLT_C128) ? 2 : 1;
+  L.meta_a.assign(plan->n_a_blks, PackMeta{0, 0, 0, 0, 0, 0});
+  L.meta_b.assign(plan->n_b_blks, PackMeta{0, 0, 0, 0, 0, 0});
+  std::vector<char> used_a(plan->n_a_blks, 0), used_b(plan->n_b_blks, 0);
+  for (size_t o = 0; o < plan->h_segs.size(); ++o) {
+    const Seg &s = plan->h_segs[o];
+    if (s.cls != kTileMfma) continue;   // row-class segments always run natively
+    for (int32_t t = s.t0; t < s.t1; ++t) {
+      const ExecTask &e = plan->h_exec[t];
+      if (!used_a[e.a_blk]) {
+        used_a[e.a_blk] = 1;
+        PackMeta &m = L.meta_a[e.a_blk];
+        m.rows = s.m;
+        m.cols = e.k;
+        m.r_pad = static_cast<int32_t>(round_up(s.m, RPAD));
+        m.k_pad = static_cast<int32_t>(round_up(e.k, KC));
+      }
+      if (!used_b[e.b_blk]) {
+        used_b[e.b_blk] = 1;
+        PackMeta &m = L.meta_b[e.b_blk];
+        m.rows = s.n;
+        m.cols = e.k;
+        m.r_pad = static_cast<int32_t>(round_up(s.n, RPAD));
+        m.k_pad = static_cast<int32_t>(round_up(e.k, KC));
+      }
+    }
+  }
+  auto assign = [&](std::vector<PackMeta> &meta, std::vector<char> &used, int side) {
+    for (size_t i = 0; i < meta.size(); ++i) {
+      if (!used[i]) continue;
+      PackMeta &m = meta[i];
+      m.plane_off = L.plane_elems;
+      m.exp_off = L.exp_elems;
+      L.plane_elems += static_cast<int64_t>(parts) * S * m.r_pad * m.k_pad;
+      L.exp_elems += m.r_pad;
+      for (int32_t r0 = 0; r0 < m.r_pad; r0 += 64)
+        L.items.push_back(PackItem{side, static_cast<int32_t>(i), r0, 0});
+    }
+  };
+  assign(L.meta_a, used_a, 0);
+  assign(L.meta_b, used_b, 1);
+  auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+  size_t off = 0;
+  L.off_meta_a = off;
+  off = align(off + sizeof(PackMeta) * L.meta_a.size());
+  L.off_meta_b = off;
+  off = align(off + sizeof(PackMeta) * L.meta_b.size());
+  L.off_items = off;
+  off = align(off + sizeof(PackItem) * L.items.size());
+  L.off_aoffs = off;
+  off = align(off + sizeof(int64_t) * L.meta_a.size());
+  L.off_boffs = off;
+  off = align(off + sizeof(int64_t) * L.meta_b.size());
+  L.off_exps = off;
+  off = align(off + sizeof(int32_t) * L.exp_elems);
+  L.off_planes = off;
+  off = align(off + sizeof(uint16_t) * L.plane_elems);
+  L.total = off;
+  return L;
+}
+
+}  // namespace
 
 namespace qlt {
 
+qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+                         const double *alpha, const double *beta, hipStream_t st);
+qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class);
+
+// Flop-weighted mean k below which a plan runs natively even in split mode:
+// such contractions are bound by memory traffic, not by the MFMA rate, and
+// slicing them would only add traffic.
+static double g_split_min_mean_k = 48.0;
+
+bool split_profitable(const qlt_plan *plan) {
+  if (plan->cls_tiles[kTileMfma] == 0) return false;
+  double w = 0.0, wk = 0.0;
+  for (const Seg &s : plan->h_segs) {
+    if (s.cls != kTileMfma) continue;
+    for (int32_t t = s.t0; t < s.t1; ++t) {
+      const double k = plan->h_exec[t].k;
+      const double f = static_cast<double>(s.m) * s.n * k;
+      w += f;
+      wk += f * k;
+    }
+  }
+  return w > 0.0 && wk / w >= g_split_min_mean_k;
+}
+
 qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *bytes) {
-  (void)plan;
-  (void)dtype;
-  *bytes = 0;
+  if (!split_profitable(plan)) {
+    *bytes = 0;
+    return QLT_OK;
+  }
+  *bytes = compute_layout(plan, dtype).total;
   return QLT_OK;
 }
 
-qlt_status launch_split(const qlt_plan *, int32_t, const void *, const void *, void *, const double *,
-                        const double *, hipStream_t) {
-  return fail(QLT_ERR_UNSUPPORTED, "split-BF16 mode is not built yet");
+qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
+  // Writes the pack metadata into the (already allocated) workspace.
+  if (!split_profitable(plan)) return QLT_OK;
+  SplitLayout L = compute_layout(plan, dtype);
+  if (plan->work_bytes < L.total) return fail(QLT_ERR_INTERNAL, "split workspace too small");
+  char *base = static_cast<char *>(plan->d_work);
+  std::vector<int64_t> aoffs(plan->n_a_blks, 0), boffs(plan->n_b_blks, 0);
+  for (const ExecTask &e : plan->h_exec) {
+    aoffs[e.a_blk] = e.a_off;
+    boffs[e.b_blk] = e.b_off;
+  }
+  QLT_HIP(hipMemcpy(base + L.off_meta_a, L.meta_a.data(), sizeof(PackMeta) * L.meta_a.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_meta_b, L.meta_b.data(), sizeof(PackMeta) * L.meta_b.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_items, L.items.data(), sizeof(PackItem) * L.items.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_aoffs, aoffs.data(), sizeof(int64_t) * aoffs.size(), hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_boffs, boffs.data(), sizeof(int64_t) * boffs.size(), hipMemcpyHostToDevice));
+  plan->split_dtype = dtype;
+  plan->split_items = static_cast<int64_t>(L.items.size());
+  const size_t offs[7] = {L.off_meta_a, L.off_meta_b, L.off_items, L.off_aoffs, L.off_boffs, L.off_exps,
+                          L.off_planes};
+  std::memcpy(plan->split_off, offs, sizeof(offs));
+  return QLT_OK;
+}
+
+qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+                        const double *alpha, const double *beta, hipStream_t st) {
+  if (!split_profitable(plan)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
+  if (plan->split_dtype != dtype)
+    return fail(QLT_ERR_INVALID_ARG, "split workspace was prepared for another dtype; call qlt_plan_prepare");
+  if (plan->d_work == nullptr) return fail(QLT_ERR_INTERNAL, "split workspace missing");
+  char *base = static_cast<char *>(plan->d_work);
+  const size_t *so = plan->split_off;
+  const auto *meta_a = reinterpret_cast<const PackMeta *>(base + so[0]);
+  const auto *meta_b = reinterpret_cast<const PackMeta *>(base + so[1]);
+  const auto *items = reinterpret_cast<const PackItem *>(base + so[2]);
+  const auto *aoffs = reinterpret_cast<const int64_t *>(base + so[3]);
+  const auto *boffs = reinterpret_cast<const int64_t *>(base + so[4]);
+  auto *exps = reinterpret_cast<int32_t *>(base + so[5]);
+  auto *planes = reinterpret_cast<uint16_t *>(base + so[6]);
+  const unsigned n_items = static_cast<unsigned>(plan->split_items);
+  const unsigned n_mfma = static_cast<unsigned>(plan->cls_tiles[kTileMfma]);
+  const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
+  const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
+  const dim3 blk(256);
+  if (dtype == QLT_F64) {
+    hipLaunchKernelGGL(k_pack<false>, dim3(n_items), blk, 0, st, items, meta_a,
+                       meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
+                       static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+    hipLaunchKernelGGL(k_split_gemm<false>, dim3(n_mfma), blk, 0, st,
+                       plan->d_tiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
+                       static_cast<double *>(c), ar, ai, br, bi);
+  } else {
+    hipLaunchKernelGGL(k_pack<true>, dim3(n_items), blk, 0, st, items, meta_a,
+                       meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
+                       static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+    hipLaunchKernelGGL(k_split_gemm<true>, dim3(n_mfma), blk, 0, st,
+                       plan->d_tiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
+                       static_cast<double *>(c), ar, ai, br, bi);
+  }
+  QLT_HIP(hipGetLastError());
+  // narrow outputs: the native row-streaming kernel
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, false);
 }
 
 }  // namespace qlt
+
+extern "C" {
+
+qlt_status qlt_set_split_min_mean_k(double min_mean_k) {
+  QLT_CHECK_ARG(min_mean_k >= 0.0, "qlt_set_split_min_mean_k: negative threshold");
+  qlt::g_split_min_mean_k = min_mean_k;
+  return QLT_OK;
+}
+
+double qlt_get_split_min_mean_k(void) { return qlt::g_split_min_mean_k; }
+
+}  // extern "C"

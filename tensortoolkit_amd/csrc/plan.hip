@@ -325,8 +325,12 @@ __global__ void k_segs(const int64_t *c_first, const int64_t *c_off, const int32
   int64_t kt = 0;
   for (int32_t t = s.t0; t < s.t1; ++t) kt += exec[t].k;
   s.k_total = kt;
+  s.cls = (n <= kRowsMaxN) ? kTileRows : kTileMfma;
+  s.pad = 0;
   segs[o] = s;
-  tile_cnt[o] = ((m + args.tile_m - 1) / args.tile_m) * ((n + args.tile_n - 1) / args.tile_n);
+  tile_cnt[o] = (s.cls == kTileRows)
+                    ? (m + kRowsPerTile - 1) / kRowsPerTile
+                    : ((m + args.tile_m - 1) / args.tile_m) * ((n + args.tile_n - 1) / args.tile_n);
   cost[o] = kt;
 }
 
@@ -336,16 +340,17 @@ __global__ void k_tiles(const Seg *segs, const int64_t *tile_start, const int64_
   const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (o >= n_c) return;
   const Seg s = segs[o];
-  const int ntn = (s.n + args.tile_n - 1) / args.tile_n;
+  const int ntn = (s.cls == kTileRows) ? 1 : (s.n + args.tile_n - 1) / args.tile_n;
   const int64_t base = tile_start[o];
   for (int64_t q = 0; q < tile_cnt[o]; ++q) {
     Tile t;
     t.seg = static_cast<int32_t>(o);
     t.tm = static_cast<int32_t>(q / ntn);
     t.tn = static_cast<int32_t>(q % ntn);
-    t.cls = 0;
+    t.cls = s.cls;
     tiles[base + q] = t;
-    tkey[base + q] = static_cast<uint64_t>(kmax - s.k_total);  // longest first
+    // class-major, then longest first inside a class
+    tkey[base + q] = (static_cast<uint64_t>(s.cls) << 40) | static_cast<uint64_t>(kmax - s.k_total);
     tval[base + q] = static_cast<int32_t>(base + q);
   }
 }
@@ -742,6 +747,8 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   QLT_HIP(hipMemcpyAsync(plan->tasks.data(), d_out, sizeof(qlt_task) * nt, hipMemcpyDeviceToHost, st));
   plan->h_segs.resize(n_c);
   QLT_HIP(hipMemcpyAsync(plan->h_segs.data(), plan->d_segs, sizeof(Seg) * n_c, hipMemcpyDeviceToHost, st));
+  plan->h_exec.resize(nt);
+  QLT_HIP(hipMemcpyAsync(plan->h_exec.data(), plan->d_exec, sizeof(ExecTask) * nt, hipMemcpyDeviceToHost, st));
   std::vector<int64_t> h_cidx(n_c), h_coff(n_c);
   std::vector<int32_t> h_ccoords(n_c * QLT_MAX_RANK), h_cdims(n_c * QLT_MAX_RANK);
   QLT_HIP(hipMemcpyAsync(h_cidx.data(), d_cidx, 8 * n_c, hipMemcpyDeviceToHost, st));
@@ -753,9 +760,17 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   const int64_t n_tiles = scal[3];
   QLT_CHECK_ARG(n_tiles < (int64_t(1) << 31), "qlt_plan_build: more than 2^31 output tiles");
 
-  // Tiles, ordered by decreasing work so the longest start first.
+  // Tiles, grouped by class and ordered by decreasing work inside a class
+  // so the longest start first.
   int64_t kmax = 0;
-  for (const Seg &s : plan->h_segs) kmax = std::max(kmax, s.k_total);
+  for (const Seg &s : plan->h_segs) {
+    kmax = std::max(kmax, s.k_total);
+    const int64_t cnt = (s.cls == kTileRows)
+                            ? (s.m + kRowsPerTile - 1) / kRowsPerTile
+                            : ((s.m + args.tile_m - 1) / args.tile_m) * ((s.n + args.tile_n - 1) / args.tile_n);
+    plan->cls_tiles[s.cls] += cnt;
+  }
+  QLT_CHECK_ARG(plan->cls_tiles[0] + plan->cls_tiles[1] == n_tiles, "qlt_plan_build: tile count mismatch");
   Tile *d_tiles_raw;
   uint64_t *d_tk0, *d_tk1;
   int32_t *d_tv0, *d_tv1;
@@ -766,7 +781,8 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   QLT_TRY(pool.alloc(&d_tv1, n_tiles));
   hipLaunchKernelGGL(k_tiles, dim3(blocks_for(n_c)), dim3(256), 0, st, plan->d_segs, d_tstart, d_tcnt, n_c,
                      kmax, args, d_tk0, d_tv0, d_tiles_raw);
-  const int tpasses = (bits_for(static_cast<uint64_t>(kmax)) + 7) / 8;
+  QLT_CHECK_ARG(kmax < (int64_t(1) << 40), "qlt_plan_build: contracted extent of an output block exceeds 2^40");
+  const int tpasses = (bits_for((static_cast<uint64_t>(kNumTileClasses - 1) << 40) | static_cast<uint64_t>(kmax)) + 7) / 8;
   if (tpasses > 0)
     hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(1024), 0, st, d_tk0, d_tv0, d_tk1, d_tv1, n_tiles,
                        tpasses);
@@ -840,9 +856,7 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
 }
 
 qlt_status qlt_plan_destroy(qlt_plan *plan) {
-  if (plan == nullptr) return QLT_OK;
-  free_plan_device(plan);
-  delete plan;
+  delete plan;   // the destructor releases the device work list
   return QLT_OK;
 }
 

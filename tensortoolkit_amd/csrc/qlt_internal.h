@@ -80,7 +80,18 @@ struct Seg {
   int32_t t0;      // first ExecTask
   int32_t t1;      // one past the last ExecTask
   int64_t k_total; // sum of k over the segment's tasks
+  int32_t cls;     // tile class of the segment (kTileMfma or kTileRows)
+  int32_t pad;
 };
+
+// Tile classes.  kTileMfma: 64 x 64 output tiles on the MFMA kernels.
+// kTileRows: narrow outputs (n <= kRowsMaxN), 256 output rows per
+// workgroup on a row-streaming kernel (bandwidth-bound work).
+constexpr int32_t kTileMfma = 0;
+constexpr int32_t kTileRows = 1;
+constexpr int32_t kNumTileClasses = 2;
+constexpr int32_t kRowsMaxN = 16;
+constexpr int32_t kRowsPerTile = 256;
 
 // One output tile of the grouped kernel.
 struct Tile {
@@ -132,10 +143,24 @@ struct qlt_plan {
   qlt::BlkGather *d_ga = nullptr;
   qlt::BlkGather *d_gb = nullptr;
   std::vector<qlt::Seg> h_segs;         // host mirror (used to rebuild offsets)
+  std::vector<qlt::ExecTask> h_exec;    // host mirror of the execution list
   // tile-size class table chosen at build time
   int32_t tile_m = 64;
   int32_t tile_n = 64;
+  int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0};  // tiles per class (class 0 first in d_tiles)
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;
+  int32_t split_dtype = -1;             // dtype the split metadata was written for
+  int64_t split_items = 0;              // pack work items
+  size_t split_off[7] = {0, 0, 0, 0, 0, 0, 0};  // workspace sub-buffer offsets
+
+  qlt_plan() = default;
+  qlt_plan(const qlt_plan &) = delete;
+  qlt_plan &operator=(const qlt_plan &) = delete;
+  ~qlt_plan() {
+    for (void *p : {static_cast<void *>(d_exec), static_cast<void *>(d_segs), static_cast<void *>(d_tiles),
+                    static_cast<void *>(d_ga), static_cast<void *>(d_gb), d_work})
+      if (p != nullptr) (void)hipFree(p);
+  }
 };

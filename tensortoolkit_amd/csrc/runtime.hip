@@ -56,19 +56,18 @@ using namespace qlt;
 // ---------------------------------------------------------------------------
 namespace {
 
-__device__ __forceinline__ double splitmix_uniform(uint64_t seed, uint64_t i) {
+__device__ __forceinline__ double splitmix_unit(uint64_t seed, uint64_t i) {
   uint64_t z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
-  const double u = static_cast<double>(z >> 11) * 0x1.0p-53;
-  return 2.0 * u - 1.0;
+  return static_cast<double>(z >> 11) * 0x1.0p-53;
 }
 
-__global__ void k_fill_uniform(double *x, int64_t n, uint64_t seed, int64_t first) {
+__global__ void k_fill_uniform(double *x, int64_t n, uint64_t seed, int64_t first, double lo, double width) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
-    x[i] = splitmix_uniform(seed, static_cast<uint64_t>(first + i));
+    x[i] = lo + width * splitmix_unit(seed, static_cast<uint64_t>(first + i));
 }
 
 __global__ void k_zero(uint8_t *p, int64_t bytes) {
@@ -356,13 +355,13 @@ qlt_status qlt_graph_destroy(qlt_graph *graph) {
 }
 
 qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t first_index,
-                            void *stream) {
+                            double lo, double hi, void *stream) {
   QLT_REQUIRE_INIT();
   QLT_CHECK_ARG(n_doubles >= 0 && first_index >= 0, "qlt_fill_uniform: negative size");
   if (n_doubles == 0) return QLT_OK;
   QLT_CHECK_ARG(x != nullptr, "qlt_fill_uniform: null buffer");
   hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(n_doubles)), dim3(256), 0, as_stream(stream),
-                     static_cast<double *>(x), n_doubles, seed, first_index);
+                     static_cast<double *>(x), n_doubles, seed, first_index, lo, hi - lo);
   QLT_HIP(hipGetLastError());
   return QLT_OK;
 }

@@ -80,6 +80,21 @@ def test_values_random(qlt, seed, dtype):
         assert per_block_rel_err(c, exp) <= TOL
 
 
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_values_random_wide_blocks(qlt, seed, dtype):
+    """Degeneracies up to 40, so many output blocks have n > 16 and take
+    the MFMA tile class (narrower ones take the row class)."""
+    rng = np.random.default_rng(7500 + seed)
+    a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1 + seed % 2, max_rank=3, max_sectors=3,
+                                   max_deg=40, min_deg=8, allow_scalar=False)
+    _say(f"case: wide seed={seed} {dtype} ranks={a.rank},{b.rank} axes={a_ax},{b_ax}")
+    plan, c = gpu_contract(a, b, a_ax, b_ax)
+    ref = O.OraclePlan(a, b, a_ax, b_ax)
+    assert_plan_equal(plan, ref)
+    assert per_block_rel_err(c, ref.execute()) <= TOL
+
+
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_large_blocks_and_ragged_edges(qlt, dtype):
     """Blocks bigger than one 64x64 tile with ragged edges and long k."""
@@ -136,7 +151,7 @@ def test_no_matching_blocks_gives_empty_plan(qlt):
 def test_device_fill_matches_host(qlt):
     n = 10007
     buf = capi.DeviceBuffer(8 * n)
-    capi.check(capi.lib().qlt_fill_uniform(buf.ptr, n, 99, 3, None))
+    capi.check(capi.lib().qlt_fill_uniform(buf.ptr, n, 99, 3, -1.0, 1.0, None))
     capi.check(capi.lib().qlt_device_sync())
     np.testing.assert_array_equal(buf.to_numpy(np.float64), O.fill_uniform(n, 99, 3))
 

@@ -1,0 +1,146 @@
+"""GPU parity of the split-BF16 (FP64-emulating) path against the FP64 oracle.
+
+Every output block must be within 1e-12 relative Frobenius error of the
+oracle, and the plan is the same plan as the native
+path.  The split kernel is forced (dispatch threshold 0) so that small test
+cases exercise it rather than the native fallback for low-k contractions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tensortoolkit_amd import capi, workloads as W
+from tensortoolkit_amd.blocksparse import BlockSparseTensor, ContractionPlan, Index, IN, OUT
+from tests.helpers import per_block_rel_err, random_case, rel_frobenius
+from tests.test_contract_gpu import gpu_contract
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SPLIT = capi.QLT_MODE_SPLIT_BF16
+
+
+@pytest.fixture
+def forced_split(qlt):
+    lib = capi.lib()
+    old = lib.qlt_get_split_min_mean_k()
+    capi.check(lib.qlt_set_split_min_mean_k(0.0))
+    yield
+    capi.check(lib.qlt_set_split_min_mean_k(old))
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_random(forced_split, seed, dtype):
+    rng = np.random.default_rng(9000 + seed)
+    a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1 + seed % 2, max_rank=3, max_sectors=3,
+                                   max_deg=40, min_deg=8, allow_scalar=False)
+    _say(f"case: split seed={seed} {dtype} ranks={a.rank},{b.rank} axes={a_ax},{b_ax}")
+    plan, c = gpu_contract(a, b, a_ax, b_ax, mode=SPLIT)
+    ref = O.OraclePlan(a, b, a_ax, b_ax)
+    err = per_block_rel_err(c, ref.execute())
+    assert err <= TOL, err
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_long_k_many_flushes(forced_split, dtype):
+    """k = 3 * 200 = 600 spans many FP32 flush intervals; ragged m, n."""
+    ix_l = Index(((0,), (1,)), (70, 131), IN)
+    ix_r = Index(((0,), (1,)), (200, 67), OUT)
+    ix_p = Index(((0,),), (3,), IN)
+    a = BlockSparseTensor.allowed_blocks([ix_l, ix_p, ix_r], (0,), dtype).fill_uniform(21)
+    b = BlockSparseTensor.allowed_blocks([ix_r.inverse(), ix_p, ix_l.inverse()], (0,), dtype).fill_uniform(22)
+    for axes in (([2, 1], [0, 1]), ([0], [2])):
+        _say(f"case: split long-k {dtype} axes={axes}")
+        plan, c = gpu_contract(a, b, *axes, mode=SPLIT)
+        ref = O.OraclePlan(a, b, *axes)
+        err = per_block_rel_err(c, ref.execute())
+        assert err <= TOL, (axes, err)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_wide_dynamic_range(forced_split, dtype):
+    """Rows of A and columns of B scaled by 2^-30 .. 2^30: the per-row and
+    per-column power-of-two scaling keeps every output block accurate."""
+    ix_m = Index(((0,),), (40,), IN)
+    ix_k = Index(((0,),), (90,), OUT)
+    ix_n = Index(((0,),), (50,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix_m, ix_k], (0,), dtype).fill_uniform(31)
+    b = BlockSparseTensor.allowed_blocks([ix_k.inverse(), ix_n], (0,), dtype).fill_uniform(32)
+    rs = 2.0 ** np.linspace(-30, 30, 40)
+    cs = 2.0 ** np.linspace(30, -30, 50)
+    a.data = (a.data.reshape(40, 90) * rs[:, None]).ravel()
+    b.data = (b.data.reshape(90, 50) * cs[None, :]).ravel()
+    plan, c = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    exp = O.OraclePlan(a, b, [1], [0]).execute()
+    # Each output row (and column) is an inner product against one scaled row
+    # of A (column of B), so with per-row / per-column scaling every row and
+    # column is accurate on its own (truncation near 2^-45 of its own scale);
+    # block-wide scaling would leave the rows scaled by 2^-30 with errors
+    # near 2^-45 * 2^30 ~ 1e-4.
+    got = c.data.reshape(40, 50)
+    ref = exp.reshape(40, 50)
+    for i in range(40):
+        assert rel_frobenius(got[i], ref[i]) <= 1e-11, ("row", i)
+    for j in range(50):
+        assert rel_frobenius(got[:, j], ref[:, j]) <= 1e-11, ("col", j)
+    assert rel_frobenius(c.data, exp) <= TOL
+
+
+def test_split_zero_blocks_and_rows(forced_split):
+    ix = Index(((0,),), (70,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix.inverse(), ix], (0,)).fill_uniform(41)
+    b = BlockSparseTensor.allowed_blocks([ix.inverse(), ix], (0,)).fill_uniform(42)
+    a.data = a.data.copy()
+    a.data.reshape(70, 70)[5:17, :] = 0.0      # zero rows take exponent 0
+    plan, c = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    exp = O.OraclePlan(a, b, [1], [0]).execute()
+    assert rel_frobenius(c.data, exp) <= TOL
+    assert np.all(c.data.reshape(70, 70)[5:17, :] == 0.0)
+
+
+def test_split_beta_and_alpha(forced_split):
+    rng = np.random.default_rng(5)
+    a, b, a_ax, b_ax = random_case(rng, dtype="c128", allow_scalar=False, max_deg=8)
+    plan = ContractionPlan(a, b, a_ax, b_ax)
+    n = plan.info.c_raw_size
+    init = (np.arange(n) % 7 - 3.0) + 1j * (np.arange(n) % 5 - 2.0)
+    da, db = capi.DeviceBuffer.from_numpy(a.data), capi.DeviceBuffer.from_numpy(b.data)
+    dc = capi.DeviceBuffer.from_numpy(init.astype(np.complex128))
+    alpha, beta = 0.5 - 1.5j, 2.0 + 0.25j
+    plan.execute("c128", SPLIT, da.ptr, db.ptr, dc.ptr, alpha=alpha, beta=beta)
+    got = dc.to_numpy(np.complex128)
+    exp = alpha * O.OraclePlan(a, b, a_ax, b_ax).execute() + beta * init
+    assert rel_frobenius(got, exp) <= TOL
+
+
+@pytest.mark.parametrize("u1u1,dtype", [(False, "f64"), (True, "c128")])
+def test_split_small_matvec_chain(forced_split, u1u1, dtype):
+    mv = W.small_matvec(D=64 if u1u1 else 96, dtype=dtype, u1u1=u1u1)
+    _say(f"case: split small matvec {mv.name}")
+    x = x_ref = mv.psi
+    for step, (ops, axes) in enumerate(zip((mv.L, mv.W, mv.W, mv.R), W.MATVEC_AXES)):
+        first, second = (ops, x) if step == 0 else (x, ops)
+        first_ref, second_ref = (ops, x_ref) if step == 0 else (x_ref, ops)
+        plan, c = gpu_contract(first, second, list(axes[0]), list(axes[1]), mode=SPLIT)
+        ref = O.OraclePlan(first_ref, second_ref, list(axes[0]), list(axes[1]))
+        c_ref = ref.output_tensor(dtype)
+        c_ref.data = ref.execute()
+        err = per_block_rel_err(c, c_ref.data)
+        assert err <= TOL, (step, err)
+        x, x_ref = c, c_ref
+
+
+def test_split_default_dispatch_keeps_low_k_native(qlt):
+    """With the default threshold a k=2 contraction runs natively in split
+    mode (bit-identical to the native result)."""
+    ix = Index(((0,), (1,)), (30, 40), IN)
+    p = Index(((0,),), (2,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix, p], (0,)).fill_uniform(1)
+    b = BlockSparseTensor.allowed_blocks([p.inverse(), ix.inverse()], (0,)).fill_uniform(2)
+    _, c_native = gpu_contract(a, b, [1], [0])
+    _, c_split = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    np.testing.assert_array_equal(c_native.data, c_split.data)
