@@ -114,7 +114,10 @@ qlt_status qlt_stream_create(void **stream);
 qlt_status qlt_stream_destroy(void *stream);
 qlt_status qlt_stream_sync(void *stream);
 
-/* Device memory behind QLMalloc / QLFree / QLMemcpy (mem_ops.h:38-104). */
+/* Device memory behind QLMalloc / QLFree / QLMemcpy (mem_ops.h:38-104).
+ * The copies are complete when they return (so work queued afterwards on any
+ * stream sees them); they do not wait for work queued earlier on non-null
+ * streams, which the caller synchronises first. */
 qlt_status qlt_malloc(void **ptr, size_t bytes);
 qlt_status qlt_free(void *ptr);
 qlt_status qlt_memcpy_h2d(void *dst, const void *src, size_t bytes);

@@ -42,15 +42,25 @@ __device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, 
   return off;
 }
 
+// Per-thread gather coordinates of the A' and B' k-chunk elements it loads.
+// A' (64 x 16): k-fast mapping -> p = tid & 15, rows (tid >> 4) + 16 q;
+//               otherwise      -> r = tid & 63, cols (tid >> 6) + 4 q.
+// B' (16 x 64): n-fast mapping -> c = tid & 63, rows (tid >> 6) + 4 q;
+//               otherwise      -> p = tid & 15, cols (tid >> 4) + 16 q.
+struct GatherPos {
+  int32_t a_fixed, b_fixed;   // offset of the fixed coordinate (row or col)
+  int32_t a_var[4], b_var[4]; // offsets of the 4 varying coordinates
+};
+
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
     const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
     double alpha_i, double beta_r, double beta_i) {
-  __shared__ double As[CPLX ? 2 : 1][KC][LDP];
-  __shared__ double Bs[CPLX ? 2 : 1][KC][LDP];
-  __shared__ int32_t s_arow[TM], s_bcol[TN], s_acol[KC], s_brow[KC];
+  constexpr int P = CPLX ? 2 : 1;
+  __shared__ double As[2][P][KC][LDP];
+  __shared__ double Bs[2][P][KC][LDP];
 
   const Tile tile = tiles[blockIdx.x];
   const Seg seg = segs[tile.seg];
@@ -58,6 +68,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
+  const bool akf = L.a_kfast != 0, bnf = L.b_nfast != 0;
 
   v4d acc[2][2], acc_i[2][2];
 #pragma unroll
@@ -68,106 +79,114 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       acc_i[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int t = seg.t0; t < seg.t1; ++t) {
-    const ExecTask et = exec[t];
-    const BlkGather &gA = ga[et.a_blk];
-    const BlkGather &gB = gb[et.b_blk];
+  // fixed (per task) row offsets of the elements this thread loads
+  int32_t a_row[4], b_col[4];
+  double2 pa[4], pb[4];
+  auto task_rows = [&](const BlkGather &gA, const BlkGather &gB) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = akf ? (tid >> 4) + 16 * q : (tid & 63);
+      const int c = bnf ? (tid & 63) : (tid >> 4) + 16 * q;
+      a_row[q] = (i0 + r < seg.m) ? split_offset(i0 + r, gA.rdim, gA.rstr, L.a_nr) : -1;
+      b_col[q] = (j0 + c < seg.n) ? split_offset(j0 + c, gB.cdim, gB.cstr, L.b_nc) : -1;
+    }
+  };
+  auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int pa_k = kk + (akf ? (tid & 15) : (tid >> 6) + 4 * q);
+      const int pb_k = kk + (bnf ? (tid >> 6) + 4 * q : (tid & 15));
+      const int32_t ac = (pa_k < et.k) ? split_offset(pa_k, gA.cdim, gA.cstr, L.a_nc) : -1;
+      const int32_t br = (pb_k < et.k) ? split_offset(pb_k, gB.rdim, gB.rstr, L.b_nr) : -1;
+      const bool oka = (a_row[q] >= 0) && (ac >= 0);
+      const bool okb = (b_col[q] >= 0) && (br >= 0);
+      const int64_t ea = et.a_off + a_row[q] + ac;
+      const int64_t eb = et.b_off + br + b_col[q];
+      if (CPLX) {
+        pa[q] = oka ? reinterpret_cast<const double2 *>(A)[ea] : double2{0.0, 0.0};
+        pb[q] = okb ? reinterpret_cast<const double2 *>(B)[eb] : double2{0.0, 0.0};
+      } else {
+        pa[q] = double2{oka ? A[ea] : 0.0, 0.0};
+        pb[q] = double2{okb ? B[eb] : 0.0, 0.0};
+      }
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ar = akf ? (tid >> 4) + 16 * q : (tid & 63);
+      const int ap = akf ? (tid & 15) : (tid >> 6) + 4 * q;
+      const int bc = bnf ? (tid & 63) : (tid >> 4) + 16 * q;
+      const int bp = bnf ? (tid >> 6) + 4 * q : (tid & 15);
+      As[buf][0][ap][ar] = pa[q].x;
+      Bs[buf][0][bp][bc] = pb[q].x;
+      if (CPLX) {
+        As[buf][P - 1][ap][ar] = pa[q].y;
+        Bs[buf][P - 1][bp][bc] = pb[q].y;
+      }
+    }
+  };
+
+  // ---- pipeline over the (task, k-chunk) sequence of this output block ----
+  int t = seg.t0, kk = 0;
+  ExecTask et = exec[t];
+  task_rows(ga[et.a_blk], gb[et.b_blk]);
+  fetch(et, ga[et.a_blk], gb[et.b_blk], 0);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  while (t < seg.t1) {
+    int t2 = t, kk2 = kk + KC;
+    if (kk2 >= et.k) {
+      t2 = t + 1;
+      kk2 = 0;
+    }
+    const bool has_next = t2 < seg.t1;
+    ExecTask et2 = et;
+    if (has_next) {
+      if (t2 != t) {
+        et2 = exec[t2];
+        task_rows(ga[et2.a_blk], gb[et2.b_blk]);
+      }
+      fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2);
+    }
+    // ---- MFMA on buffer `buf`: each wave 32 x 32 as 2 x 2 tiles of 16 x 16 x 4 ----
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 4) {
+      const double a0 = As[buf][0][ks + lk][wm + lr];
+      const double a1 = As[buf][0][ks + lk][wm + 16 + lr];
+      const double b0 = Bs[buf][0][ks + lk][wn + lr];
+      const double b1 = Bs[buf][0][ks + lk][wn + 16 + lr];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      if (CPLX) {
+        const double ai0 = As[buf][P - 1][ks + lk][wm + lr];
+        const double ai1 = As[buf][P - 1][ks + lk][wm + 16 + lr];
+        const double bi0 = Bs[buf][P - 1][ks + lk][wn + lr];
+        const double bi1 = Bs[buf][P - 1][ks + lk][wn + 16 + lr];
+        // re += ar*br - ai*bi ; im += ar*bi + ai*br
+        acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi0, acc_i[0][0], 0, 0, 0);
+        acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi1, acc_i[0][1], 0, 0, 0);
+        acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi0, acc_i[1][0], 0, 0, 0);
+        acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi1, acc_i[1][1], 0, 0, 0);
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai0, bi0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai0, bi1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai1, bi0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai1, bi1, acc[1][1], 0, 0, 0);
+        acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b0, acc_i[0][0], 0, 0, 0);
+        acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b1, acc_i[0][1], 0, 0, 0);
+        acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b0, acc_i[1][0], 0, 0, 0);
+        acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b1, acc_i[1][1], 0, 0, 0);
+      }
+    }
+    if (has_next) stage(buf ^ 1);
     __syncthreads();
-    if (tid < TM) {
-      const int r = i0 + tid;
-      s_arow[tid] = (r < seg.m) ? split_offset(r, gA.rdim, gA.rstr, L.a_nr) : -1;
-    } else if (tid < TM + TN) {
-      const int c = j0 + tid - TM;
-      s_bcol[tid - TM] = (c < seg.n) ? split_offset(c, gB.cdim, gB.cstr, L.b_nc) : -1;
-    }
-    for (int kk = 0; kk < et.k; kk += KC) {
-      if (tid >= 128 && tid < 128 + KC) {
-        const int p = kk + tid - 128;
-        s_acol[tid - 128] = (p < et.k) ? split_offset(p, gA.cdim, gA.cstr, L.a_nc) : -1;
-      } else if (tid >= 192 && tid < 192 + KC) {
-        const int p = kk + tid - 192;
-        s_brow[tid - 192] = (p < et.k) ? split_offset(p, gB.rdim, gB.rstr, L.b_nr) : -1;
-      }
-      __syncthreads();
-      // ---- gather A'(i0.., kk..) -> As[p][r] ----
-#pragma unroll
-      for (int q = 0; q < (TM * KC) / 256; ++q) {
-        int r, p;
-        if (L.a_kfast) {
-          p = tid & (KC - 1);
-          r = (tid >> 4) + 16 * q;
-        } else {
-          r = tid & (TM - 1);
-          p = (tid >> 6) + 4 * q;
-        }
-        const int32_t ro = s_arow[r], co = s_acol[p];
-        const bool ok = (ro >= 0) && (co >= 0);
-        const int64_t e = et.a_off + ro + co;
-        if (CPLX) {
-          double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
-          As[0][p][r] = v.x;
-          As[1][p][r] = v.y;
-        } else {
-          As[0][p][r] = ok ? A[e] : 0.0;
-        }
-      }
-      // ---- gather B'(kk.., j0..) -> Bs[p][c] ----
-#pragma unroll
-      for (int q = 0; q < (TN * KC) / 256; ++q) {
-        int c, p;
-        if (L.b_nfast) {
-          c = tid & (TN - 1);
-          p = (tid >> 6) + 4 * q;
-        } else {
-          p = tid & (KC - 1);
-          c = (tid >> 4) + 16 * q;
-        }
-        const int32_t ro = s_brow[p], co = s_bcol[c];
-        const bool ok = (ro >= 0) && (co >= 0);
-        const int64_t e = et.b_off + ro + co;
-        if (CPLX) {
-          double2 v = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
-          Bs[0][p][c] = v.x;
-          Bs[1][p][c] = v.y;
-        } else {
-          Bs[0][p][c] = ok ? B[e] : 0.0;
-        }
-      }
-      __syncthreads();
-      // ---- MFMA: each wave 32 x 32 as 2 x 2 tiles of 16 x 16 x 4 ----
-#pragma unroll
-      for (int ks = 0; ks < KC; ks += 4) {
-        const double a0 = As[0][ks + lk][wm + lr];
-        const double a1 = As[0][ks + lk][wm + 16 + lr];
-        const double b0 = Bs[0][ks + lk][wn + lr];
-        const double b1 = Bs[0][ks + lk][wn + 16 + lr];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        if (CPLX) {
-          const double ai0 = As[1][ks + lk][wm + lr];
-          const double ai1 = As[1][ks + lk][wm + 16 + lr];
-          const double bi0 = Bs[1][ks + lk][wn + lr];
-          const double bi1 = Bs[1][ks + lk][wn + 16 + lr];
-          const double nai0 = -ai0, nai1 = -ai1;
-          // re += ar*br - ai*bi ; im += ar*bi + ai*br
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai0, bi0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai0, bi1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai1, bi0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai1, bi1, acc[1][1], 0, 0, 0);
-          acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi0, acc_i[0][0], 0, 0, 0);
-          acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi1, acc_i[0][1], 0, 0, 0);
-          acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi0, acc_i[1][0], 0, 0, 0);
-          acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi1, acc_i[1][1], 0, 0, 0);
-          acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b0, acc_i[0][0], 0, 0, 0);
-          acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b1, acc_i[0][1], 0, 0, 0);
-          acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b0, acc_i[1][0], 0, 0, 0);
-          acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b1, acc_i[1][1], 0, 0, 0);
-        }
-      }
-      __syncthreads();
-    }
+    t = t2;
+    kk = kk2;
+    et = et2;
+    buf ^= 1;
   }
 
   // ---- epilogue: C = alpha * acc + beta * C (C untouched when beta == 0) ----
@@ -201,82 +220,65 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       }
 }
 
-// Row-streaming kernel for narrow outputs (n <= kRowsMaxN): one thread per
-// output row, 256 rows per workgroup.  Each k-chunk of A' (256 rows x 16) is
-// staged in LDS with loads contiguous along A's innermost memory axis (rows
-// or k), the matching 16 x n chunk of B' is staged once per workgroup, and
-// every thread accumulates its n outputs with FP64 FMAs.  The work is
-// bound by reading A and writing C once.
+// Row-streaming kernel for narrow outputs (n <= NMAX): one thread per output
+// row, RPT rows per thread (rows tid, tid + 256, ...).  A' is read straight
+// from global memory -- consecutive rows are consecutive threads, so the
+// loads are contiguous whenever A's innermost axis is a saved one, and with
+// RPT independent rows each thread keeps several loads in flight.  The
+// 16 x n chunk of B' is staged once per workgroup in LDS and every thread
+// accumulates its n outputs with FP64 FMAs.  Bound by reading A and writing
+// C once.
 constexpr int RK = 16;            // k per stage
-constexpr int RLD = kRowsPerTile + 1;
 
-template <bool CPLX>
+template <bool CPLX, int NMAX, int RPT>
 __global__ __launch_bounds__(256) void k_rows_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
     const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
     double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
-  __shared__ double As[P][RK][RLD];
-  __shared__ double Bs[P][RK][kRowsMaxN];
-  __shared__ int32_t s_arow[kRowsPerTile];
-  __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[kRowsMaxN];
+  static_assert(RK * NMAX <= 256, "one B' element per thread");
+  __shared__ double Bs[P][RK][NMAX];
+  __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[NMAX];
 
   const Tile tile = tiles[blockIdx.x];
   const Seg seg = segs[tile.seg];
   const int tid = threadIdx.x;
-  const int r0 = tile.tm * kRowsPerTile;
-  const int row = r0 + tid;
+  const int r0 = tile.tm * (256 * RPT);
   const int n = seg.n;
 
-  double acc[kRowsMaxN], acc_i[kRowsMaxN];
+  double acc[RPT][NMAX], acc_i[RPT][NMAX];
 #pragma unroll
-  for (int j = 0; j < kRowsMaxN; ++j) {
-    acc[j] = 0.0;
-    acc_i[j] = 0.0;
-  }
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      acc[q][j] = 0.0;
+      acc_i[q][j] = 0.0;
+    }
 
   for (int t = seg.t0; t < seg.t1; ++t) {
     const ExecTask et = exec[t];
     const BlkGather &gA = ga[et.a_blk];
     const BlkGather &gB = gb[et.b_blk];
+    int32_t roff[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int row = r0 + tid + 256 * q;
+      roff[q] = (row < seg.m) ? split_offset(row, gA.rdim, gA.rstr, L.a_nr) : -1;
+    }
     __syncthreads();
-    s_arow[tid] = (row < seg.m) ? split_offset(row, gA.rdim, gA.rstr, L.a_nr) : -1;
-    if (tid < kRowsMaxN) s_bcol[tid] = (tid < n) ? split_offset(tid, gB.cdim, gB.cstr, L.b_nc) : -1;
+    if (tid < NMAX) s_bcol[tid] = (tid < n) ? split_offset(tid, gB.cdim, gB.cstr, L.b_nc) : -1;
     for (int kk = 0; kk < et.k; kk += RK) {
-      if (tid >= 128 && tid < 128 + RK) {
-        const int p = kk + tid - 128;
-        s_acol[tid - 128] = (p < et.k) ? split_offset(p, gA.cdim, gA.cstr, L.a_nc) : -1;
-      } else if (tid >= 192 && tid < 192 + RK) {
-        const int p = kk + tid - 192;
-        s_brow[tid - 192] = (p < et.k) ? split_offset(p, gB.rdim, gB.rstr, L.b_nr) : -1;
+      if (tid < RK) {
+        const int p = kk + tid;
+        s_acol[tid] = (p < et.k) ? split_offset(p, gA.cdim, gA.cstr, L.a_nc) : -1;
+      } else if (tid >= 64 && tid < 64 + RK) {
+        const int p = kk + tid - 64;
+        s_brow[tid - 64] = (p < et.k) ? split_offset(p, gB.rdim, gB.rstr, L.b_nr) : -1;
       }
       __syncthreads();
-      // A' chunk: 256 rows x RK
-#pragma unroll 4
-      for (int q = 0; q < RK; ++q) {
-        int r, p;
-        if (L.a_kfast) {
-          p = tid & (RK - 1);
-          r = (tid >> 4) + 16 * q;
-        } else {
-          r = tid;
-          p = q;
-        }
-        const int32_t ro = s_arow[r], co = s_acol[p];
-        const bool ok = (ro >= 0) && (co >= 0);
-        const int64_t e = et.a_off + ro + co;
-        if (CPLX) {
-          const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
-          As[0][p][r] = v.x;
-          As[P - 1][p][r] = v.y;
-        } else {
-          As[0][p][r] = ok ? A[e] : 0.0;
-        }
-      }
-      // B' chunk: RK x n
-      {
-        const int p = tid / kRowsMaxN, j = tid % kRowsMaxN;
+      if (tid < RK * NMAX) {
+        const int p = tid / NMAX, j = tid % NMAX;
         const int32_t ro = s_brow[p], co = s_bcol[j];
         const bool ok = (ro >= 0) && (co >= 0);
         const int64_t e = et.b_off + ro + co;
@@ -290,21 +292,38 @@ __global__ __launch_bounds__(256) void k_rows_f64(
       }
       __syncthreads();
       const int kc = min(RK, et.k - kk);
+#pragma unroll 2
       for (int p = 0; p < kc; ++p) {
-        const double a = As[0][p][tid];
-        const double a_im = As[P - 1][p][tid];
+        const int32_t co = s_acol[p];
+        double av[RPT], av_i[RPT];
 #pragma unroll
-        for (int j = 0; j < kRowsMaxN; ++j) {
+        for (int q = 0; q < RPT; ++q) {
+          const bool ok = roff[q] >= 0;
+          const int64_t e = et.a_off + roff[q] + co;
+          if (CPLX) {
+            const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
+            av[q] = v.x;
+            av_i[q] = v.y;
+          } else {
+            av[q] = ok ? A[e] : 0.0;
+            av_i[q] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
           if (j < n) {
             const double b = Bs[0][p][j];
-            if (CPLX) {
-              const double b_im = Bs[P - 1][p][j];
-              acc[j] = fma(a, b, acc[j]);
-              acc[j] = fma(-a_im, b_im, acc[j]);
-              acc_i[j] = fma(a, b_im, acc_i[j]);
-              acc_i[j] = fma(a_im, b, acc_i[j]);
-            } else {
-              acc[j] = fma(a, b, acc[j]);
+            const double b_im = Bs[P - 1][p][j];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+              if (CPLX) {
+                acc[q][j] = fma(av[q], b, acc[q][j]);
+                acc[q][j] = fma(-av_i[q], b_im, acc[q][j]);
+                acc_i[q][j] = fma(av[q], b_im, acc_i[q][j]);
+                acc_i[q][j] = fma(av_i[q], b, acc_i[q][j]);
+              } else {
+                acc[q][j] = fma(av[q], b, acc[q][j]);
+              }
             }
           }
         }
@@ -313,25 +332,30 @@ __global__ __launch_bounds__(256) void k_rows_f64(
     }
   }
 
-  if (row >= seg.m) return;
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
-  const int64_t base = seg.c_off + static_cast<int64_t>(row) * n;
 #pragma unroll
-  for (int j = 0; j < kRowsMaxN; ++j) {
-    if (j < n) {
-      if (CPLX) {
-        double yr = alpha_r * acc[j] - alpha_i * acc_i[j], yi = alpha_r * acc_i[j] + alpha_i * acc[j];
-        double2 *cp = reinterpret_cast<double2 *>(C) + base + j;
-        if (use_beta) {
-          const double2 o = *cp;
-          yr += beta_r * o.x - beta_i * o.y;
-          yi += beta_r * o.y + beta_i * o.x;
+  for (int q = 0; q < RPT; ++q) {
+    const int row = r0 + tid + 256 * q;
+    if (row >= seg.m) continue;
+    const int64_t base = seg.c_off + static_cast<int64_t>(row) * n;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      if (j < n) {
+        if (CPLX) {
+          double yr = alpha_r * acc[q][j] - alpha_i * acc_i[q][j];
+          double yi = alpha_r * acc_i[q][j] + alpha_i * acc[q][j];
+          double2 *cp = reinterpret_cast<double2 *>(C) + base + j;
+          if (use_beta) {
+            const double2 o = *cp;
+            yr += beta_r * o.x - beta_i * o.y;
+            yi += beta_r * o.y + beta_i * o.x;
+          }
+          *cp = double2{yr, yi};
+        } else {
+          double y = alpha_r * acc[q][j];
+          if (use_beta) y += beta_r * C[base + j];
+          C[base + j] = y;
         }
-        *cp = double2{yr, yi};
-      } else {
-        double y = alpha_r * acc[j];
-        if (use_beta) y += beta_r * C[base + j];
-        C[base + j] = y;
       }
     }
   }
@@ -349,7 +373,7 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 block(256);
-  const int64_t n0 = plan->cls_tiles[kTileMfma], n1 = plan->cls_tiles[kTileRows];
+  const int64_t n0 = plan->cls_tiles[kTileMfma];
   const auto *A = static_cast<const double *>(a);
   const auto *B = static_cast<const double *>(b);
   auto *Cp = static_cast<double *>(c);
@@ -361,14 +385,24 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
       hipLaunchKernelGGL(k_grouped_gemm_f64<true>, dim3(static_cast<unsigned>(n0)), block, 0, st, plan->d_tiles,
                          plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
   }
+  const int64_t n1 = plan->cls_tiles[kTileRows4], n2 = plan->cls_tiles[kTileRows16];
   if (n1 > 0) {
     const Tile *t1 = plan->d_tiles + n0;
     if (dtype == QLT_F64)
-      hipLaunchKernelGGL(k_rows_f64<false>, dim3(static_cast<unsigned>(n1)), block, 0, st, t1, plan->d_segs,
-                         plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+      hipLaunchKernelGGL((k_rows_f64<false, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
     else
-      hipLaunchKernelGGL(k_rows_f64<true>, dim3(static_cast<unsigned>(n1)), block, 0, st, t1, plan->d_segs,
-                         plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+      hipLaunchKernelGGL((k_rows_f64<true, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+  }
+  if (n2 > 0) {
+    const Tile *t2 = plan->d_tiles + n0 + n1;
+    if (dtype == QLT_F64)
+      hipLaunchKernelGGL((k_rows_f64<false, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+    else
+      hipLaunchKernelGGL((k_rows_f64<true, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
+                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
   }
   QLT_HIP(hipGetLastError());
   return QLT_OK;

@@ -261,7 +261,12 @@ __device__ __forceinline__ constexpr int group_of(int sa, int sb) {
 }
 
 // Split GEMM: one 256-thread workgroup per 64 x 64 output tile, 2 x 2 waves of
-// 32 x 32, one 32x32x16 MFMA block per slice product.
+// 32 x 32, one 32x32x16 MFMA block per slice product.  The (task, k-chunk)
+// sequence of the tile's output block is walked as one software pipeline:
+// while the MFMAs consume the chunk staged in one LDS buffer, the next
+// chunk's planes (possibly of the next task) are already in flight into
+// registers, and are written to the other LDS buffer afterwards, so there
+// is a single barrier per chunk and the load latency hides behind MFMA work.
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_split_gemm(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
@@ -271,9 +276,11 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   constexpr int PARTS = CPLX ? 2 : 1;
   constexpr int KCH = CPLX ? 32 : 64;   // k between flushes (keeps levels 2..4 exact)
   constexpr int NOUT = CPLX ? 2 : 1;
-  __shared__ __attribute__((aligned(16))) uint16_t As[PARTS][S][TM][LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[PARTS][S][TN][LDK];
-  __shared__ int32_t s_ea[TM];
+  constexpr int CHUNKS = PARTS * S * 128;          // 16-byte pieces per operand per k-chunk
+  constexpr int PER_THREAD = (CHUNKS + 255) / 256;
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][PARTS][S][TM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
+  __shared__ int32_t s_ea[2][TM];
 
   const Tile tile = tiles[blockIdx.x];
   const Seg seg = segs[tile.seg];
@@ -293,40 +300,80 @@ __global__ __launch_bounds__(256) void k_split_gemm(
 #pragma unroll
     for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
 
-  for (int t = seg.t0; t < seg.t1; ++t) {
-    const ExecTask et = exec[t];
-    const PackMeta ma = meta_a[et.a_blk];
-    const PackMeta mb = meta_b[et.b_blk];
-    __syncthreads();
-    if (tid < TM) s_ea[tid] = exps[ma.exp_off + i0 + tid];
-    const int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  v4u pa[PER_THREAD], pb[PER_THREAD];
+  int32_t ea_next = 0;
+  // issue the global loads of chunk kc of the task described by (ma, mb)
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
     const int nkc = ma.k_pad / KC;
-    int since_flush = 0;
-    for (int kc = 0; kc < nkc; ++kc) {
-      // ---- stage the KC-chunk of every slice plane of A and B in LDS ----
-      // per (part, s): 64 rows x 16 bf16 = 2 KB contiguous = 128 x 16 B
-      constexpr int CHUNKS = PARTS * S * 128;
-      for (int c = tid; c < CHUNKS; c += 256) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + 256 * u;
+      if (c < CHUNKS) {
         const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
         const int64_t ia = ma.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * ma.r_pad + i0 + rr) * KC + half * 8;
         const int64_t ib = mb.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * mb.r_pad + j0 + rr) * KC + half * 8;
-        *reinterpret_cast<v4u *>(&As[ps / S][ps % S][rr][half * 8]) = *reinterpret_cast<const v4u *>(planes + ia);
-        *reinterpret_cast<v4u *>(&Bs[ps / S][ps % S][rr][half * 8]) = *reinterpret_cast<const v4u *>(planes + ib);
+        pa[u] = *reinterpret_cast<const v4u *>(planes + ia);
+        pb[u] = *reinterpret_cast<const v4u *>(planes + ib);
       }
-      __syncthreads();
-      // ---- 15 slice products (x4 real products for complex) ----
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + 256 * u;
+      if (c < CHUNKS) {
+        const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+        *reinterpret_cast<v4u *>(&As[buf][ps / S][ps % S][rr][half * 8]) = pa[u];
+        *reinterpret_cast<v4u *>(&Bs[buf][ps / S][ps % S][rr][half * 8]) = pb[u];
+      }
+    }
+  };
+
+  // ---- pipeline prologue: chunk 0 of the first task ----
+  int t = seg.t0, kc = 0;
+  PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
+  int nkc = ma.k_pad / KC;
+  int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  fetch(ma, mb, 0);
+  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + i0 + tid];
+  stage(0);
+  __syncthreads();
+  int buf = 0, since_flush = 0;
+
+  while (t < seg.t1) {
+    // ---- prefetch the next chunk (same task or the next one) ----
+    int t2 = t, kc2 = kc + 1;
+    if (kc2 == nkc) {
+      t2 = t + 1;
+      kc2 = 0;
+    }
+    const bool has_next = t2 < seg.t1;
+    PackMeta ma2 = ma, mb2 = mb;
+    int32_t eb2 = eb;
+    if (has_next) {
+      if (t2 != t) {
+        const ExecTask e2 = exec[t2];
+        ma2 = meta_a[e2.a_blk];
+        mb2 = meta_b[e2.b_blk];
+        eb2 = exps[mb2.exp_off + j0 + wn + lrow];
+        if (tid < TM) ea_next = exps[ma2.exp_off + i0 + tid];
+      }
+      fetch(ma2, mb2, kc2);
+    }
+    // ---- 15 slice products (x4 real products for complex) on buffer `buf` ----
+    {
       v8bf bfr[PARTS][S];
 #pragma unroll
       for (int part = 0; part < PARTS; ++part)
 #pragma unroll
-        for (int s = 0; s < S; ++s)
-          bfr[part][s] = *reinterpret_cast<const v8bf *>(&Bs[part][s][wn + lrow][lk]);
+        for (int s2 = 0; s2 < S; ++s2)
+          bfr[part][s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][part][s2][wn + lrow][lk]);
 #pragma unroll
       for (int sa = 0; sa < S; ++sa) {
-        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[0][sa][wm + lrow][lk]);
-        v8bf ai, nai;
+        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][0][sa][wm + lrow][lk]);
+        v8bf ai = ar, nai = ar;
         if (CPLX) {
-          ai = *reinterpret_cast<const v8bf *>(&As[PARTS - 1][sa][wm + lrow][lk]);
+          ai = *reinterpret_cast<const v8bf *>(&As[buf][PARTS - 1][sa][wm + lrow][lk]);
           nai = neg8(ai);
         }
 #pragma unroll
@@ -334,34 +381,49 @@ __global__ __launch_bounds__(256) void k_split_gemm(
           const int q = group_of(sa, sb);
           g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[0][sb], g[0][q], 0, 0, 0);
           if (CPLX) {
+            g[NOUT - 1][q] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[PARTS - 1][sb], g[NOUT - 1][q], 0, 0, 0);
             g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nai, bfr[PARTS - 1][sb], g[0][q], 0, 0, 0);
-            g[NOUT - 1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[PARTS - 1][sb], g[NOUT - 1][q], 0, 0, 0);
             g[NOUT - 1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, bfr[0][sb], g[NOUT - 1][q], 0, 0, 0);
           }
         }
       }
-      since_flush += KC;
-      // ---- flush to FP64 ----
-      if (since_flush >= KCH || kc == nkc - 1) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          const int e = s_ea[row] + eb - 16;
-#pragma unroll
-          for (int o = 0; o < NOUT; ++o) {
-            const double sum = static_cast<double>(g[o][0][r]) + static_cast<double>(g[o][1][r]) +
-                               static_cast<double>(g[o][2][r]) + static_cast<double>(g[o][3][r]);
-            acc64[o][r] += ldexp(sum, e);
-          }
-        }
-#pragma unroll
-        for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
-        since_flush = 0;
-      }
-      __syncthreads();
     }
+    since_flush += KC;
+    // ---- flush to FP64 (always at the end of a task: exponents change) ----
+    if (since_flush >= KCH || kc == nkc - 1) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int e = s_ea[buf][row] + eb - 16;
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+          const double sum = static_cast<double>(g[o][0][r]) + static_cast<double>(g[o][1][r]) +
+                             static_cast<double>(g[o][2][r]) + static_cast<double>(g[o][3][r]);
+          acc64[o][r] += ldexp(sum, e);
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
+      since_flush = 0;
+    }
+    // ---- stage the prefetched chunk into the other buffer ----
+    if (has_next) {
+      stage(buf ^ 1);
+      if (tid < TM) s_ea[buf ^ 1][tid] = (t2 != t) ? ea_next : s_ea[buf][tid];
+    }
+    __syncthreads();
+    if (t2 != t) {
+      ma = ma2;
+      mb = mb2;
+      nkc = ma.k_pad / KC;
+      eb = eb2;
+    }
+    t = t2;
+    kc = kc2;
+    buf ^= 1;
   }
 
   // ---- epilogue ----

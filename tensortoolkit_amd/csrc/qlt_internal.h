@@ -80,18 +80,23 @@ struct Seg {
   int32_t t0;      // first ExecTask
   int32_t t1;      // one past the last ExecTask
   int64_t k_total; // sum of k over the segment's tasks
-  int32_t cls;     // tile class of the segment (kTileMfma or kTileRows)
+  int32_t cls;     // tile class of the segment (kTileMfma, kTileRows4, kTileRows16)
   int32_t pad;
 };
 
 // Tile classes.  kTileMfma: 64 x 64 output tiles on the MFMA kernels.
-// kTileRows: narrow outputs (n <= kRowsMaxN), 256 output rows per
-// workgroup on a row-streaming kernel (bandwidth-bound work).
+// kTileRows4 / kTileRows16: narrow outputs (n <= 4 / n <= 16) on a
+// row-streaming kernel (bandwidth-bound work), 1024 / 256 output rows per
+// workgroup (4 / 1 rows per thread).
 constexpr int32_t kTileMfma = 0;
-constexpr int32_t kTileRows = 1;
-constexpr int32_t kNumTileClasses = 2;
-constexpr int32_t kRowsMaxN = 16;
-constexpr int32_t kRowsPerTile = 256;
+constexpr int32_t kTileRows4 = 1;
+constexpr int32_t kTileRows16 = 2;
+constexpr int32_t kNumTileClasses = 3;
+
+__host__ __device__ constexpr int32_t tile_class_of(int64_t n) {
+  return n <= 4 ? kTileRows4 : (n <= 16 ? kTileRows16 : kTileMfma);
+}
+__host__ __device__ constexpr int32_t rows_per_tile(int32_t cls) { return cls == kTileRows4 ? 1024 : 256; }
 
 // One output tile of the grouped kernel.
 struct Tile {
@@ -147,7 +152,7 @@ struct qlt_plan {
   // tile-size class table chosen at build time
   int32_t tile_m = 64;
   int32_t tile_n = 64;
-  int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0};  // tiles per class (class 0 first in d_tiles)
+  int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0};  // tiles per class, class-major in d_tiles
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;

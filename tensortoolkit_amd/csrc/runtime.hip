@@ -279,6 +279,7 @@ qlt_status qlt_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   QLT_REQUIRE_INIT();
   if (bytes == 0) return QLT_OK;
   QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  QLT_HIP(hipStreamSynchronize(nullptr));   // complete on return, for callers on other streams
   return QLT_OK;
 }
 
@@ -286,6 +287,7 @@ qlt_status qlt_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   QLT_REQUIRE_INIT();
   if (bytes == 0) return QLT_OK;
   QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  QLT_HIP(hipStreamSynchronize(nullptr));   // complete on return, for callers on other streams
   return QLT_OK;
 }
 
@@ -293,6 +295,7 @@ qlt_status qlt_memcpy_d2d(void *dst, const void *src, size_t bytes) {
   QLT_REQUIRE_INIT();
   if (bytes == 0) return QLT_OK;
   QLT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  QLT_HIP(hipStreamSynchronize(nullptr));   // complete on return, for callers on other streams
   return QLT_OK;
 }
 

@@ -1,0 +1,338 @@
+// Tests of the qlten host API (include/qlten/) on the device, written the way
+// the reference tests contraction (tests/test_tensor_manipulation/
+// test_ten_ctrct.cc): U(1) indexes with sectors {-1, 0, +1} of degeneracy 3
+// ("s") or 10 ("l"), Random tensors, each result checked element by element
+// against a dense contraction computed on the host; path equivalence of the
+// contiguous-axes API against the general Contract; accumulate semantics.
+// Exit status 0 means every check passed.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <functional>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "qlten/qlten.h"
+
+using namespace qlten;
+using U1Idx = Index<U1QN>;
+using QNSctT = QNSector<U1QN>;
+using DTen = QLTensor<QLTEN_Double, U1QN>;
+using ZTen = QLTensor<QLTEN_Complex, U1QN>;
+
+static int g_fail = 0;
+static int g_checks = 0;
+#define CHECK(cond, msg)                                                     \
+  do {                                                                       \
+    ++g_checks;                                                              \
+    if (!(cond)) {                                                           \
+      ++g_fail;                                                              \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, std::string(msg).c_str()); \
+    }                                                                        \
+  } while (0)
+
+// Dense host copy of a tensor (row-major over the full index dimensions).
+template <typename T>
+std::vector<T> Densify(const QLTensor<T, U1QN> &t) {
+  size_t total = 1;
+  for (size_t d : t.GetShape()) total *= d;
+  std::vector<T> out(total, T(0));
+  if (t.IsDefault()) return out;
+  if (t.IsScalar()) {
+    auto raw = t.GetBlkSparDataTen().RawDataToHost();
+    if (!raw.empty()) out[0] = raw[0];
+    return out;
+  }
+  auto raw = t.GetBlkSparDataTen().RawDataToHost();
+  const size_t r = t.Rank();
+  std::vector<std::vector<size_t>> soff(r);
+  for (size_t i = 0; i < r; ++i) {
+    size_t acc = 0;
+    for (size_t s = 0; s < t.GetIndex(i).GetQNSctNum(); ++s) {
+      soff[i].push_back(acc);
+      acc += t.GetIndex(i).GetQNSct(s).dim();
+    }
+  }
+  for (const auto &kv : t.GetBlkSparDataTen().GetBlkIdxDataBlkMap()) {
+    const auto &blk = kv.second;
+    std::vector<size_t> c(r, 0);
+    for (size_t e = 0; e < blk.size; ++e) {
+      size_t rem = e, lin = 0;
+      for (int i = static_cast<int>(r) - 1; i >= 0; --i) {
+        c[i] = rem % blk.shape[i];
+        rem /= blk.shape[i];
+      }
+      for (size_t i = 0; i < r; ++i) lin = lin * t.GetShape()[i] + soff[i][blk.blk_coors[i]] + c[i];
+      out[lin] = raw[blk.data_offset + e];
+    }
+  }
+  return out;
+}
+
+// Dense contraction of A and B over the given axes; result axes are A's saved
+// (in a_saved order) then B's saved (in b_saved order).
+template <typename T>
+std::vector<T> DenseContract(const std::vector<T> &a, const ShapeT &sa, const std::vector<T> &b, const ShapeT &sb,
+                             const std::vector<size_t> &a_ax, const std::vector<size_t> &b_ax,
+                             const std::vector<size_t> &a_sv, const std::vector<size_t> &b_sv) {
+  size_t m = 1, n = 1, k = 1;
+  for (size_t x : a_sv) m *= sa[x];
+  for (size_t x : b_sv) n *= sb[x];
+  for (size_t x : a_ax) k *= sa[x];
+  std::vector<T> c(m * n, T(0));
+  auto strides = [](const ShapeT &s) {
+    std::vector<size_t> st(s.size(), 1);
+    for (int i = static_cast<int>(s.size()) - 2; i >= 0; --i) st[i] = st[i + 1] * s[i + 1];
+    return st;
+  };
+  const auto ast = strides(sa), bst = strides(sb);
+  auto offset = [](size_t lin, const std::vector<size_t> &axes, const ShapeT &shape, const std::vector<size_t> &st) {
+    size_t off = 0;
+    for (int j = static_cast<int>(axes.size()) - 1; j >= 0; --j) {
+      off += (lin % shape[axes[j]]) * st[axes[j]];
+      lin /= shape[axes[j]];
+    }
+    return off;
+  };
+  for (size_t i = 0; i < m; ++i) {
+    const size_t ai = offset(i, a_sv, sa, ast);
+    for (size_t j = 0; j < n; ++j) {
+      const size_t bj = offset(j, b_sv, sb, bst);
+      T s = T(0);
+      for (size_t p = 0; p < k; ++p) {
+        size_t ap = 0, bp = 0, rem = p;
+        for (int q = static_cast<int>(a_ax.size()) - 1; q >= 0; --q) {
+          const size_t c = rem % sa[a_ax[q]];
+          rem /= sa[a_ax[q]];
+          ap += c * ast[a_ax[q]];
+          bp += c * bst[b_ax[q]];
+        }
+        s += a[ai + ap] * b[bj + bp];
+      }
+      c[i * n + j] = s;
+    }
+  }
+  return c;
+}
+
+template <typename T>
+double RelDiff(const std::vector<T> &x, const std::vector<T> &y) {
+  double num = 0, den = 0;
+  for (size_t i = 0; i < x.size(); ++i) {
+    num += std::norm(std::complex<double>(x[i] - y[i]));
+    den += std::norm(std::complex<double>(y[i]));
+  }
+  return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+}
+
+struct Fixture {
+  U1QN qn0 = U1QN(0), qnp1 = U1QN(1), qnm1 = U1QN(-1);
+  U1Idx idx_in_s, idx_out_s, idx_in_l, idx_out_l;
+  Fixture() {
+    QNSectorVec<U1QN> s = {QNSctT(qnm1, 3), QNSctT(qn0, 3), QNSctT(qnp1, 3)};
+    QNSectorVec<U1QN> l = {QNSctT(qnm1, 10), QNSctT(qn0, 10), QNSctT(qnp1, 10)};
+    idx_in_s = U1Idx(s, IN);
+    idx_out_s = U1Idx(s, OUT);
+    idx_in_l = U1Idx(l, IN);
+    idx_out_l = U1Idx(l, OUT);
+  }
+};
+
+template <typename T>
+void CheckContract(const QLTensor<T, U1QN> &a, const QLTensor<T, U1QN> &b, const std::vector<size_t> &a_ax,
+                   const std::vector<size_t> &b_ax, const std::string &name) {
+  QLTensor<T, U1QN> c;
+  Contract(&a, &b, {a_ax, b_ax}, &c);
+  std::vector<size_t> a_sv, b_sv;
+  for (size_t i = 0; i < a.Rank(); ++i)
+    if (std::find(a_ax.begin(), a_ax.end(), i) == a_ax.end()) a_sv.push_back(i);
+  for (size_t i = 0; i < b.Rank(); ++i)
+    if (std::find(b_ax.begin(), b_ax.end(), i) == b_ax.end()) b_sv.push_back(i);
+  const auto ref = DenseContract(Densify(a), a.GetShape(), Densify(b), b.GetShape(), a_ax, b_ax, a_sv, b_sv);
+  const auto got = Densify(c);
+  CHECK(got.size() == ref.size(), name + ": size");
+  const double err = RelDiff(got, ref);
+  CHECK(err < 1e-13, name + ": rel err " + std::to_string(err));
+  CHECK(c.Rank() == a_sv.size() + b_sv.size(), name + ": rank");
+}
+
+int main() {
+  Fixture f;
+  SetRandomSeed(7);
+
+  // ---- general Contract, real, rank 1..3, every div (test_ten_ctrct.cc:226-494 pattern) ----
+  for (int div = -1; div <= 1; ++div) {
+    DTen t1({f.idx_out_s});
+    t1.Random(U1QN(div));
+    DTen t1d({f.idx_in_s});
+    t1d.Random(U1QN(div));
+    CheckContract(t1, t1d, {0}, {0}, "rank1 full div " + std::to_string(div));
+
+    DTen a2({f.idx_in_s, f.idx_out_l});
+    DTen b2({f.idx_in_l, f.idx_out_s});
+    a2.Random(U1QN(div));
+    b2.Random(U1QN(0));
+    CheckContract(a2, b2, {1}, {0}, "rank2 tail-head");
+    DTen b2t({f.idx_out_s, f.idx_in_l});
+    b2t.Random(U1QN(div));
+    CheckContract(a2, b2t, {1}, {1}, "rank2 tail-tail");
+
+    DTen a3({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+    DTen b3({f.idx_in_l, f.idx_in_s, f.idx_out_l});
+    a3.Random(U1QN(div));
+    b3.Random(U1QN(div));
+    CheckContract(a3, b3, {1}, {0}, "rank3 one axis");
+    CheckContract(a3, b3, {1, 2}, {0, 1}, "rank3 two axes");
+    CheckContract(a3, b3, {2, 1}, {1, 0}, "rank3 two axes reversed");
+  }
+
+  // ---- complex and mixed real x complex (ten_ctrct.h:292-350) ----
+  {
+    ZTen a({f.idx_in_l, f.idx_out_s, f.idx_out_l});
+    ZTen b({f.idx_in_l, f.idx_in_s, f.idx_out_s});
+    a.Random(U1QN(0));
+    b.Random(U1QN(1));
+    CheckContract(a, b, {2}, {0}, "complex rank3");
+    CheckContract(a, b, {2, 1}, {0, 1}, "complex rank3 two axes");
+    DTen ar({f.idx_in_l, f.idx_out_s, f.idx_out_l});
+    ar.Random(U1QN(0));
+    ZTen c_mixed;
+    Contract(&ar, &b, {{2}, {0}}, &c_mixed);
+    ZTen c_promoted;
+    auto arz = ToComplex(ar);
+    Contract(&arz, &b, {{2}, {0}}, &c_promoted);
+    CHECK(RelDiff(Densify(c_mixed), Densify(c_promoted)) == 0.0, "mixed real x complex == promoted");
+  }
+
+  // ---- t . dagger(t) = sum |t_i|^2 (test_ten_ctrct.cc:187-206) ----
+  {
+    ZTen t({f.idx_out_l});
+    t.Random(U1QN(0));
+    ZTen td = Dag(t);
+    ZTen s;
+    Contract(&t, &td, {{0}, {0}}, &s);
+    double ref = 0;
+    for (auto v : Densify(t)) ref += std::norm(v);
+    const auto got = s.GetElem({});
+    CHECK(std::abs(got - QLTEN_Complex(ref, 0.0)) < 1e-12 * ref, "t.dag(t)");
+  }
+
+  // ---- contiguous-axes path == general Contract (test_ten_ctrct.cc:638-859) ----
+  {
+    DTen a({f.idx_in_s, f.idx_out_l, f.idx_out_s, f.idx_in_l});
+    DTen b({f.idx_in_s, f.idx_out_l, f.idx_in_l});
+    a.Random(U1QN(0));
+    b.Random(U1QN(1));
+    // A axes 2,3 with B axes 0,1 (Tail/Head): saved A = (0,1), B = (2)
+    DTen c_gen, c_cont;
+    Contract(&a, &b, {{2, 3}, {0, 1}}, &c_gen);
+    ContiguousContractStats stats;
+    ContractContiguousAxes<QLTEN_Double, U1QN, CtrctSide::Tail, CtrctSide::Head>(a, b, 2, 0, 2, c_cont, &stats);
+    CHECK(c_gen.GetIndexes() == c_cont.GetIndexes(), "contiguous indexes");
+    CHECK(RelDiff(Densify(c_cont), Densify(c_gen)) < 1e-13, "contiguous tail-head values");
+    CHECK(stats.raw_data_contract_tasks > 0, "stats tasks");
+    // wrap-around: A axes (3, 0) contracted -> saved cyclic (1, 2)
+    DTen a2({f.idx_out_s, f.idx_out_l, f.idx_in_s, f.idx_in_l});
+    a2.Random(U1QN(0));
+    DTen b2({f.idx_out_l, f.idx_in_s, f.idx_out_s});
+    b2.Random(U1QN(0));
+    DTen c_wrap, c_ref;
+    ContractContiguousAxes<QLTEN_Double, U1QN, CtrctSide::Head, CtrctSide::Head>(a2, b2, 3, 0, 2, c_wrap);
+    Contract(&a2, &b2, {{3, 0}, {0, 1}}, &c_ref);   // general: saved A ascending (1, 2) -- same order here
+    CHECK(RelDiff(Densify(c_wrap), Densify(c_ref)) < 1e-13, "contiguous wrap-around");
+    // legacy bool form
+    DTen c_legacy;
+    Contract<QLTEN_Double, U1QN, true, true>(a, b, 2, 0, 2, c_legacy);
+    CHECK(RelDiff(Densify(c_legacy), Densify(c_gen)) < 1e-13, "legacy bool Contract");
+  }
+
+  // ---- accumulate: C = beta C + alpha A.B (contract_contiguous_axes.h:664-782) ----
+  {
+    DTen a({f.idx_in_l, f.idx_out_l});
+    DTen b({f.idx_in_l, f.idx_out_s});
+    a.Random(U1QN(0));
+    b.Random(U1QN(0));
+    DTen c0;
+    Contract(&a, &b, {{1}, {0}}, &c0);
+    DTen acc(c0);
+    ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, 2.0, 0.5, acc);
+    auto expect = Densify(c0);
+    for (auto &v : expect) v = 2.5 * v;   // 0.5 * C0 + 2 * A.B with C0 = A.B
+    CHECK(RelDiff(Densify(acc), expect) < 1e-13, "accumulate alpha/beta");
+    DTen fresh;
+    ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, 3.0, 0.0, fresh);
+    auto e3 = Densify(c0);
+    for (auto &v : e3) v *= 3.0;
+    CHECK(RelDiff(Densify(fresh), e3) < 1e-13, "accumulate into default output");
+    bool threw = false;
+    try {
+      DTen empty;
+      ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, 1.0, 1.0, empty);
+    } catch (const std::invalid_argument &) {
+      threw = true;
+    }
+    CHECK(threw, "beta != 0 on a default output throws");
+    threw = false;
+    try {
+      ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, 1.0, 1.0, a);
+    } catch (const std::invalid_argument &) {
+      threw = true;
+    }
+    CHECK(threw, "aliasing throws");
+  }
+
+  // ---- plan cache reuse on a repeated pattern ----
+  {
+    DTen a({f.idx_in_l, f.idx_out_l});
+    DTen b({f.idx_in_l, f.idx_out_l});
+    a.Random(U1QN(0));
+    b.Random(U1QN(0));
+    DTen c1, c2;
+    const size_t h0 = detail::PlanCache::Instance().hits;
+    Contract(&a, &b, {{1}, {0}}, &c1);
+    a.Random(U1QN(0));   // same topology, new values
+    Contract(&a, &b, {{1}, {0}}, &c2);
+    CHECK(detail::PlanCache::Instance().hits > h0, "plan cache hit");
+    CheckContract(a, b, {1}, {0}, "after cache hit");
+  }
+
+  // ---- split-BF16 precision mode keeps 1e-12 ----
+  {
+    QNSectorVec<U1QN> big = {QNSctT(U1QN(-1), 70), QNSctT(U1QN(0), 90), QNSctT(U1QN(1), 80)};
+    U1Idx bi(big, IN), bo(big, OUT);
+    ZTen a({bi, bo});
+    ZTen b({bi, bo});
+    a.Random(U1QN(0));
+    b.Random(U1QN(0));
+    ZTen cn, cs;
+    Contract(&a, &b, {{1}, {0}}, &cn);
+    QltCheck(qlt_set_split_min_mean_k(0.0));
+    hp_numeric::SetContractPrecision(hp_numeric::ContractPrecision::kSplitBF16);
+    Contract(&a, &b, {{1}, {0}}, &cs);
+    hp_numeric::SetContractPrecision(hp_numeric::ContractPrecision::kNativeFP64);
+    QltCheck(qlt_set_split_min_mean_k(48.0));
+    const double err = RelDiff(Densify(cs), Densify(cn));
+    CHECK(err < 1e-12, "split mode rel err " + std::to_string(err));
+  }
+
+  // ---- element access, norms, linear combination ----
+  {
+    DTen t({f.idx_in_s, f.idx_out_s});
+    t.Random(U1QN(0));
+    const double n2 = t.Get2Norm();
+    double ref = 0;
+    for (double v : Densify(t)) ref += v * v;
+    CHECK(std::abs(n2 - std::sqrt(ref)) < 1e-12 * std::sqrt(ref), "Get2Norm");
+    t.SetElem({0, 0}, 42.0);
+    CHECK(t.GetElem({0, 0}) == 42.0, "SetElem/GetElem");
+    CHECK(t.GetElem({0, 3}) == 0.0, "element of an absent block is 0");
+    DTen u = t + t * 2.0;
+    CHECK(RelDiff(Densify(u), [&] { auto d = Densify(t); for (auto &v : d) v *= 3.0; return d; }()) < 1e-14,
+          "operator+ / operator*");
+    CHECK(t.Div() == U1QN(0), "Div");
+  }
+
+  std::printf("%d checks, %d failures\n", g_checks, g_fail);
+  return g_fail == 0 ? 0 : 1;
+}
