@@ -173,9 +173,10 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
                                 void *stream);
 
 /* Split-mode dispatch threshold: a plan whose flop-weighted mean contracted
- * extent k is below `min_mean_k` runs on the native kernel even in
- * QLT_MODE_SPLIT_BF16 (such contractions are bound by memory traffic, where
- * slicing only adds traffic).  Default 48; 0 forces the split kernel.  A
+ * extent k is below `min_mean_k`, or whose flops per byte of distinct operand
+ * blocks are below 96, runs on the native kernel even in QLT_MODE_SPLIT_BF16
+ * (such contractions are bound by memory traffic, where slicing only adds
+ * traffic).  Default 48; 0 forces the split kernel.  A
  * runtime knob in the spirit of the reference's SetCpuNumericsThreads
  * (framework/hp_numeric/omp_set.h:67-83); it has no reference counterpart. */
 qlt_status qlt_set_split_min_mean_k(double min_mean_k);

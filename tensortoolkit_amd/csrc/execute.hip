@@ -12,7 +12,7 @@ qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *by
 qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                         const double *alpha, const double *beta, hipStream_t st);
 qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype);
-bool split_profitable(const qlt_plan *plan);
+bool split_profitable(const qlt_plan *plan, int32_t dtype);
 }  // namespace qlt
 
 using namespace qlt;
@@ -25,7 +25,7 @@ qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode) {
   QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_plan_prepare: bad dtype");
   QLT_CHECK_ARG(mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16, "qlt_plan_prepare: bad mode");
   if (mode == QLT_MODE_NATIVE || plan->n_tiles == 0) return QLT_OK;
-  if (!split_profitable(plan)) return QLT_OK;
+  if (!split_profitable(plan, dtype)) return QLT_OK;
   size_t need = 0;
   QLT_TRY(split_workspace_bytes(plan, dtype, &need));
   if (need > plan->work_bytes) {
@@ -54,7 +54,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
   QLT_CHECK_ARG(a != nullptr && b != nullptr && c != nullptr, "qlt_contract_execute: null data buffer");
   hipStream_t st = as_stream(stream);
   if (mode == QLT_MODE_NATIVE) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
-  if (split_profitable(plan) && plan->split_dtype != dtype) {
+  if (split_profitable(plan, dtype) && plan->split_dtype != dtype) {
     QLT_CHECK_ARG(!stream_capturing(st),
                   "qlt_contract_execute: split workspace not prepared; call qlt_plan_prepare before capture");
     QLT_TRY(qlt_plan_prepare(const_cast<qlt_plan *>(plan), dtype, mode));

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  const bool akf = L.a_kfast != 0, bnf = L.b_nfast != 0;
+  // thread mapping of the chunk being fetched (k-contiguous for A', n-contiguous
+  // for B'), chosen per task: plain blocks always read along memory order
+  bool akf = L.a_kfast != 0, bnf = L.b_nfast != 0;
 
   v4d acc[2][2], acc_i[2][2];
 #pragma unroll
@@ -79,16 +81,29 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       acc_i[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  // fixed (per task) row offsets of the elements this thread loads
+  // Per task: the row offsets of the A' elements and column offsets of the
+  // B' elements this thread loads.  A block whose permuted view is plain
+  // (row i, col p at i * ncols + p) needs no decomposition at all.
   int32_t a_row[4], b_col[4];
+  bool a_plain = false, b_plain = false;
   double2 pa[4], pb[4];
-  auto task_rows = [&](const BlkGather &gA, const BlkGather &gB) {
+  auto task_rows = [&](const BlkGather &gA, const BlkGather &gB, const ExecTask &et) {
+    a_plain = gA.plain != 0;
+    b_plain = gB.plain != 0;
+    akf = a_plain || (L.a_kfast != 0);
+    bnf = b_plain || (L.b_nfast != 0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = akf ? (tid >> 4) + 16 * q : (tid & 63);
       const int c = bnf ? (tid & 63) : (tid >> 4) + 16 * q;
-      a_row[q] = (i0 + r < seg.m) ? split_offset(i0 + r, gA.rdim, gA.rstr, L.a_nr) : -1;
-      b_col[q] = (j0 + c < seg.n) ? split_offset(j0 + c, gB.cdim, gB.cstr, L.b_nc) : -1;
+      if (a_plain)
+        a_row[q] = (i0 + r < seg.m) ? (i0 + r) * et.k : -1;
+      else
+        a_row[q] = (i0 + r < seg.m) ? split_offset(i0 + r, gA.rdim, gA.rstr, L.a_nr) : -1;
+      if (b_plain)
+        b_col[q] = (j0 + c < seg.n) ? (j0 + c) : -1;
+      else
+        b_col[q] = (j0 + c < seg.n) ? split_offset(j0 + c, gB.cdim, gB.cstr, L.b_nc) : -1;
     }
   };
   auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk) {
@@ -96,8 +111,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     for (int q = 0; q < 4; ++q) {
       const int pa_k = kk + (akf ? (tid & 15) : (tid >> 6) + 4 * q);
       const int pb_k = kk + (bnf ? (tid >> 6) + 4 * q : (tid & 15));
-      const int32_t ac = (pa_k < et.k) ? split_offset(pa_k, gA.cdim, gA.cstr, L.a_nc) : -1;
-      const int32_t br = (pb_k < et.k) ? split_offset(pb_k, gB.rdim, gB.rstr, L.b_nr) : -1;
+      const int32_t ac = (pa_k < et.k) ? (a_plain ? pa_k : split_offset(pa_k, gA.cdim, gA.cstr, L.a_nc)) : -1;
+      const int32_t br = (pb_k < et.k) ? (b_plain ? pb_k * seg.n : split_offset(pb_k, gB.rdim, gB.rstr, L.b_nr)) : -1;
       const bool oka = (a_row[q] >= 0) && (ac >= 0);
       const bool okb = (b_col[q] >= 0) && (br >= 0);
       const int64_t ea = et.a_off + a_row[q] + ac;
@@ -130,7 +145,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   // ---- pipeline over the (task, k-chunk) sequence of this output block ----
   int t = seg.t0, kk = 0;
   ExecTask et = exec[t];
-  task_rows(ga[et.a_blk], gb[et.b_blk]);
+  task_rows(ga[et.a_blk], gb[et.b_blk], et);
   fetch(et, ga[et.a_blk], gb[et.b_blk], 0);
   stage(0);
   __syncthreads();
@@ -146,7 +161,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     if (has_next) {
       if (t2 != t) {
         et2 = exec[t2];
-        task_rows(ga[et2.a_blk], gb[et2.b_blk]);
+        task_rows(ga[et2.a_blk], gb[et2.b_blk], et2);
       }
       fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2);
     }
@@ -260,21 +275,23 @@ __global__ __launch_bounds__(256) void k_rows_f64(
     const ExecTask et = exec[t];
     const BlkGather &gA = ga[et.a_blk];
     const BlkGather &gB = gb[et.b_blk];
+    const bool a_plain = gA.plain != 0, b_plain = gB.plain != 0;
     int32_t roff[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int row = r0 + tid + 256 * q;
-      roff[q] = (row < seg.m) ? split_offset(row, gA.rdim, gA.rstr, L.a_nr) : -1;
+      roff[q] = (row < seg.m) ? (a_plain ? row * et.k : split_offset(row, gA.rdim, gA.rstr, L.a_nr)) : -1;
     }
     __syncthreads();
-    if (tid < NMAX) s_bcol[tid] = (tid < n) ? split_offset(tid, gB.cdim, gB.cstr, L.b_nc) : -1;
+    if (tid < NMAX)
+      s_bcol[tid] = (tid < n) ? (b_plain ? tid : split_offset(tid, gB.cdim, gB.cstr, L.b_nc)) : -1;
     for (int kk = 0; kk < et.k; kk += RK) {
       if (tid < RK) {
         const int p = kk + tid;
-        s_acol[tid] = (p < et.k) ? split_offset(p, gA.cdim, gA.cstr, L.a_nc) : -1;
+        s_acol[tid] = (p < et.k) ? (a_plain ? p : split_offset(p, gA.cdim, gA.cstr, L.a_nc)) : -1;
       } else if (tid >= 64 && tid < 64 + RK) {
         const int p = kk + tid - 64;
-        s_brow[tid - 64] = (p < et.k) ? split_offset(p, gB.rdim, gB.rstr, L.b_nr) : -1;
+        s_brow[tid - 64] = (p < et.k) ? (b_plain ? p * n : split_offset(p, gB.rdim, gB.rstr, L.b_nr)) : -1;
       }
       __syncthreads();
       if (tid < RK * NMAX) {

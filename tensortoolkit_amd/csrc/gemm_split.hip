@@ -118,13 +118,18 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   const int32_t *cstr = is_a ? g.cstr : g.rstr;
   const int nr = is_a ? L.a_nr : L.b_nc;
   const int nc = is_a ? L.a_nc : L.b_nr;
+  // A plain block: row r, col p at r * cols + p (k contiguous).  A plain B
+  // block seen transposed: row j, col p at p * rows + j (rows contiguous).
+  const bool plain = g.plain != 0;
   // whether consecutive k are consecutive in memory
-  const bool kfast = is_a ? (L.a_kfast != 0) : (L.b_nfast == 0);
+  const bool kfast = plain ? is_a : (is_a ? (L.a_kfast != 0) : (L.b_nfast == 0));
   const int tid = threadIdx.x;
+  auto row_off = [&](int r) { return plain ? (is_a ? r * pm.cols : r) : split_offset(r, rdim, rstr, nr); };
+  auto col_off = [&](int p) { return plain ? (is_a ? p : p * pm.rows) : split_offset(p, cdim, cstr, nc); };
 
   if (tid < 64) {
     const int r = it.r0 + tid;
-    s_roff[tid] = (r < pm.rows) ? split_offset(r, rdim, rstr, nr) : -1;
+    s_roff[tid] = (r < pm.rows) ? row_off(r) : -1;
     s_max[tid] = 0ull;
   }
   __syncthreads();
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   for (int p0 = 0; p0 < pm.cols; p0 += 64) {
     if (tid < 64) {
       const int p = p0 + tid;
-      s_coff[tid] = (p < pm.cols) ? split_offset(p, cdim, cstr, nc) : -1;
+      s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
     }
     __syncthreads();
     for (int q = 0; q < 16; ++q) {
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   for (int p0 = 0; p0 < pm.k_pad; p0 += KP) {
     if (tid < KP) {
       const int p = p0 + tid;
-      s_coff[tid] = (p < pm.cols) ? split_offset(p, cdim, cstr, nc) : -1;
+      s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
     }
     __syncthreads();
     for (int q = 0; q < (64 * KP) / 256; ++q) {
@@ -542,23 +547,43 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
 // slicing them would only add traffic.
 static double g_split_min_mean_k = 48.0;
 
-bool split_profitable(const qlt_plan *plan) {
+// Algorithmic flops per byte of the distinct operand blocks below which the
+// split path loses to the native one: packing reads every operand block
+// twice and writes 5/4 of it back as slice planes, so a contraction whose
+// operands are large relative to its flops is cheaper natively.
+constexpr double kSplitMinFlopsPerByte = 96.0;
+
+bool split_profitable(const qlt_plan *plan, int32_t dtype) {
   if (plan->cls_tiles[kTileMfma] == 0) return false;
-  double w = 0.0, wk = 0.0;
+  if (g_split_min_mean_k <= 0.0) return true;   // forced
+  double w = 0.0, wk = 0.0, elems = 0.0;
+  std::vector<char> seen_a(plan->n_a_blks, 0), seen_b(plan->n_b_blks, 0);
   for (const Seg &s : plan->h_segs) {
     if (s.cls != kTileMfma) continue;
     for (int32_t t = s.t0; t < s.t1; ++t) {
-      const double k = plan->h_exec[t].k;
+      const ExecTask &e = plan->h_exec[t];
+      const double k = e.k;
       const double f = static_cast<double>(s.m) * s.n * k;
       w += f;
       wk += f * k;
+      if (!seen_a[e.a_blk]) {
+        seen_a[e.a_blk] = 1;
+        elems += static_cast<double>(s.m) * k;
+      }
+      if (!seen_b[e.b_blk]) {
+        seen_b[e.b_blk] = 1;
+        elems += k * static_cast<double>(s.n);
+      }
     }
   }
-  return w > 0.0 && wk / w >= g_split_min_mean_k;
+  if (w <= 0.0 || wk / w < g_split_min_mean_k) return false;
+  const double flops = (dtype == QLT_C128 ? 8.0 : 2.0) * w;
+  const double bytes = (dtype == QLT_C128 ? 16.0 : 8.0) * elems;
+  return flops / bytes >= kSplitMinFlopsPerByte;
 }
 
 qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *bytes) {
-  if (!split_profitable(plan)) {
+  if (!split_profitable(plan, dtype)) {
     *bytes = 0;
     return QLT_OK;
   }
@@ -568,7 +593,7 @@ qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *by
 
 qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
   // Writes the pack metadata into the (already allocated) workspace.
-  if (!split_profitable(plan)) return QLT_OK;
+  if (!split_profitable(plan, dtype)) return QLT_OK;
   SplitLayout L = compute_layout(plan, dtype);
   if (plan->work_bytes < L.total) return fail(QLT_ERR_INTERNAL, "split workspace too small");
   char *base = static_cast<char *>(plan->d_work);
@@ -595,7 +620,7 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
 
 qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                         const double *alpha, const double *beta, hipStream_t st) {
-  if (!split_profitable(plan)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
+  if (!split_profitable(plan, dtype)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
   if (plan->split_dtype != dtype)
     return fail(QLT_ERR_INVALID_ARG, "split workspace was prepared for another dtype; call qlt_plan_prepare");
   if (plan->d_work == nullptr) return fail(QLT_ERR_INTERNAL, "split workspace missing");

@@ -380,14 +380,29 @@ __global__ void k_gather_desc(const int32_t *dims, int64_t n, const PlanArgs arg
     g.cdim[j] = 1;
     g.cstr[j] = 0;
   }
+  int32_t nrows = 1, ncols = 1, last_axis = -1, plain = 1;
   for (int j = 0; j < nr; ++j) {
     g.rdim[j] = d[raxes[j]];
     g.rstr[j] = stride[raxes[j]];
+    nrows *= d[raxes[j]];
+    if (d[raxes[j]] > 1) {
+      if (raxes[j] < last_axis) plain = 0;
+      last_axis = raxes[j];
+    }
   }
   for (int j = 0; j < nc; ++j) {
     g.cdim[j] = d[caxes[j]];
     g.cstr[j] = stride[caxes[j]];
+    ncols *= d[caxes[j]];
+    if (d[caxes[j]] > 1) {
+      if (caxes[j] < last_axis) plain = 0;
+      last_axis = caxes[j];
+    }
   }
+  g.plain = plain;
+  g.ncols = ncols;
+  g.nrows = nrows;
+  g.pad = 0;
   out[i] = g;
 }
 

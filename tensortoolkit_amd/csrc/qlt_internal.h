@@ -59,6 +59,13 @@ struct BlkGather {
   int32_t rstr[QLT_MAX_RANK];
   int32_t cdim[QLT_MAX_RANK];
   int32_t cstr[QLT_MAX_RANK];
+  // 1 when the permuted block is the block itself in memory order, i.e. the
+  // element (row i, col p) sits at i * ncols + p: the axes of extent > 1,
+  // taken in (row axes ++ col axes) order, appear in increasing memory order.
+  int32_t plain;
+  int32_t ncols;   // product of cdim (the leading dimension when plain)
+  int32_t nrows;   // product of rdim
+  int32_t pad;
 };
 
 // One task in execution order (sorted by output block, generation order
