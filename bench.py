@@ -201,6 +201,9 @@ def main() -> int:
     ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-replays", type=int, default=20)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this size first (scales rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -208,6 +211,15 @@ def main() -> int:
         return 2
 
     capi.init(0)
+    if args.calibration_copy_mb:
+        nbytes = args.calibration_copy_mb << 20
+        src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+        capi.check(capi.lib().qlt_memset_zero(src.ptr, nbytes, None))
+        capi.check(capi.lib().qlt_memcpy_d2d(dst.ptr, src.ptr, nbytes))
+        capi.check(capi.lib().qlt_device_sync())
+        print(f"[bench] calibration copy: {nbytes} bytes device to device", file=sys.stderr, flush=True)
+        src.close()
+        dst.close()
     stream = C.c_void_p()
     capi.check(capi.lib().qlt_stream_create(C.byref(stream)))
     hip = Hip()

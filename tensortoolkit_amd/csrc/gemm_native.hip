@@ -42,103 +42,106 @@ __device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, 
   return off;
 }
 
-// Per-thread gather coordinates of the A' and B' k-chunk elements it loads.
-// A' (64 x 16): k-fast mapping -> p = tid & 15, rows (tid >> 4) + 16 q;
-//               otherwise      -> r = tid & 63, cols (tid >> 6) + 4 q.
-// B' (16 x 64): n-fast mapping -> c = tid & 63, rows (tid >> 6) + 4 q;
-//               otherwise      -> p = tid & 15, cols (tid >> 4) + 16 q.
-struct GatherPos {
-  int32_t a_fixed, b_fixed;   // offset of the fixed coordinate (row or col)
-  int32_t a_var[4], b_var[4]; // offsets of the 4 varying coordinates
-};
-
-template <bool CPLX>
+// Grouped MFMA kernel, one 256-thread workgroup per TM x TN output tile
+// (TM, TN in {32, 64}); the four waves split the tile 2 x 2, each wave
+// computing (TM/2) x (TN/2) as BM x BN blocks of v_mfma_f64_16x16x4f64.
+// Per-thread gather of the A' (TM x 16) and B' (16 x TN) k-chunk:
+//   A' k-contiguous -> p = tid & 15, rows (tid >> 4) + 16 q;
+//      otherwise    -> r = tid % TM, cols tid / TM + (256 / TM) q;
+//   B' n-contiguous -> c = tid % TN, rows tid / TN + (256 / TN) q;
+//      otherwise    -> p = tid & 15, cols (tid >> 4) + 16 q.
+template <bool CPLX, int TMv, int TNv>
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
     const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
     double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
-  __shared__ double As[2][P][KC][LDP];
-  __shared__ double Bs[2][P][KC][LDP];
+  constexpr int QA = TMv / 16, QB = TNv / 16;          // elements per thread per chunk
+  constexpr int BM = TMv / 32, BN = TNv / 32;          // MFMA blocks per wave
+  __shared__ double As[2][P][KC][TMv + 1];
+  __shared__ double Bs[2][P][KC][TNv + 1];
 
   const Tile tile = tiles[blockIdx.x];
   const Seg seg = segs[tile.seg];
-  const int i0 = tile.tm * TM, j0 = tile.tn * TN;
+  const int i0 = tile.tm * TMv, j0 = tile.tn * TNv;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int wm = (w >> 1) * (TMv / 2), wn = (w & 1) * (TNv / 2);
   const int lr = lane & 15, lk = lane >> 4;
-  // thread mapping of the chunk being fetched (k-contiguous for A', n-contiguous
-  // for B'), chosen per task: plain blocks always read along memory order
+  // thread mapping of the chunk being fetched, chosen per task: plain
+  // blocks always read along memory order
   bool akf = L.a_kfast != 0, bnf = L.b_nfast != 0;
 
-  v4d acc[2][2], acc_i[2][2];
+  v4d acc[BM][BN], acc_i[BM][BN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < BM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < BN; ++b) {
       acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
+  auto a_r = [&](int q) { return akf ? (tid >> 4) + 16 * q : tid % TMv; };
+  auto a_p = [&](int q) { return akf ? (tid & 15) : tid / TMv + (256 / TMv) * q; };
+  auto b_c = [&](int q) { return bnf ? tid % TNv : (tid >> 4) + 16 * q; };
+  auto b_p = [&](int q) { return bnf ? tid / TNv + (256 / TNv) * q : (tid & 15); };
+
   // Per task: the row offsets of the A' elements and column offsets of the
   // B' elements this thread loads.  A block whose permuted view is plain
   // (row i, col p at i * ncols + p) needs no decomposition at all.
-  int32_t a_row[4], b_col[4];
+  int32_t a_row[QA], b_col[QB];
   bool a_plain = false, b_plain = false;
-  double2 pa[4], pb[4];
+  double2 pa[QA], pb[QB];
   auto task_rows = [&](const BlkGather &gA, const BlkGather &gB, const ExecTask &et) {
     a_plain = gA.plain != 0;
     b_plain = gB.plain != 0;
     akf = a_plain || (L.a_kfast != 0);
     bnf = b_plain || (L.b_nfast != 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = akf ? (tid >> 4) + 16 * q : (tid & 63);
-      const int c = bnf ? (tid & 63) : (tid >> 4) + 16 * q;
-      if (a_plain)
-        a_row[q] = (i0 + r < seg.m) ? (i0 + r) * et.k : -1;
-      else
-        a_row[q] = (i0 + r < seg.m) ? split_offset(i0 + r, gA.rdim, gA.rstr, L.a_nr) : -1;
-      if (b_plain)
-        b_col[q] = (j0 + c < seg.n) ? (j0 + c) : -1;
-      else
-        b_col[q] = (j0 + c < seg.n) ? split_offset(j0 + c, gB.cdim, gB.cstr, L.b_nc) : -1;
+    for (int q = 0; q < QA; ++q) {
+      const int r = i0 + a_r(q);
+      a_row[q] = (r < seg.m) ? (a_plain ? r * et.k : split_offset(r, gA.rdim, gA.rstr, L.a_nr)) : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int c = j0 + b_c(q);
+      b_col[q] = (c < seg.n) ? (b_plain ? c : split_offset(c, gB.cdim, gB.cstr, L.b_nc)) : -1;
     }
   };
   auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int pa_k = kk + (akf ? (tid & 15) : (tid >> 6) + 4 * q);
-      const int pb_k = kk + (bnf ? (tid >> 6) + 4 * q : (tid & 15));
-      const int32_t ac = (pa_k < et.k) ? (a_plain ? pa_k : split_offset(pa_k, gA.cdim, gA.cstr, L.a_nc)) : -1;
-      const int32_t br = (pb_k < et.k) ? (b_plain ? pb_k * seg.n : split_offset(pb_k, gB.rdim, gB.rstr, L.b_nr)) : -1;
-      const bool oka = (a_row[q] >= 0) && (ac >= 0);
-      const bool okb = (b_col[q] >= 0) && (br >= 0);
-      const int64_t ea = et.a_off + a_row[q] + ac;
-      const int64_t eb = et.b_off + br + b_col[q];
-      if (CPLX) {
-        pa[q] = oka ? reinterpret_cast<const double2 *>(A)[ea] : double2{0.0, 0.0};
-        pb[q] = okb ? reinterpret_cast<const double2 *>(B)[eb] : double2{0.0, 0.0};
-      } else {
-        pa[q] = double2{oka ? A[ea] : 0.0, 0.0};
-        pb[q] = double2{okb ? B[eb] : 0.0, 0.0};
-      }
+    for (int q = 0; q < QA; ++q) {
+      const int pk = kk + a_p(q);
+      const int32_t ac = (pk < et.k) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, L.a_nc)) : -1;
+      const bool ok = (a_row[q] >= 0) && (ac >= 0);
+      const int64_t e = et.a_off + a_row[q] + ac;
+      if (CPLX)
+        pa[q] = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
+      else
+        pa[q] = double2{ok ? A[e] : 0.0, 0.0};
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int pk = kk + b_p(q);
+      const int32_t br = (pk < et.k) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, L.b_nr)) : -1;
+      const bool ok = (b_col[q] >= 0) && (br >= 0);
+      const int64_t e = et.b_off + br + b_col[q];
+      if (CPLX)
+        pb[q] = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
+      else
+        pb[q] = double2{ok ? B[e] : 0.0, 0.0};
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int ar = akf ? (tid >> 4) + 16 * q : (tid & 63);
-      const int ap = akf ? (tid & 15) : (tid >> 6) + 4 * q;
-      const int bc = bnf ? (tid & 63) : (tid >> 4) + 16 * q;
-      const int bp = bnf ? (tid >> 6) + 4 * q : (tid & 15);
-      As[buf][0][ap][ar] = pa[q].x;
-      Bs[buf][0][bp][bc] = pb[q].x;
-      if (CPLX) {
-        As[buf][P - 1][ap][ar] = pa[q].y;
-        Bs[buf][P - 1][bp][bc] = pb[q].y;
-      }
+    for (int q = 0; q < QA; ++q) {
+      As[buf][0][a_p(q)][a_r(q)] = pa[q].x;
+      if (CPLX) As[buf][P - 1][a_p(q)][a_r(q)] = pa[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      Bs[buf][0][b_p(q)][b_c(q)] = pb[q].x;
+      if (CPLX) Bs[buf][P - 1][b_p(q)][b_c(q)] = pb[q].y;
     }
   };
 
@@ -165,35 +168,33 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       }
       fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2);
     }
-    // ---- MFMA on buffer `buf`: each wave 32 x 32 as 2 x 2 tiles of 16 x 16 x 4 ----
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
-      const double a0 = As[buf][0][ks + lk][wm + lr];
-      const double a1 = As[buf][0][ks + lk][wm + 16 + lr];
-      const double b0 = Bs[buf][0][ks + lk][wn + lr];
-      const double b1 = Bs[buf][0][ks + lk][wn + 16 + lr];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      double a[BM], b[BN];
+#pragma unroll
+      for (int mt = 0; mt < BM; ++mt) a[mt] = As[buf][0][ks + lk][wm + mt * 16 + lr];
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt) b[nt] = Bs[buf][0][ks + lk][wn + nt * 16 + lr];
+#pragma unroll
+      for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < BN; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
       if (CPLX) {
-        const double ai0 = As[buf][P - 1][ks + lk][wm + lr];
-        const double ai1 = As[buf][P - 1][ks + lk][wm + 16 + lr];
-        const double bi0 = Bs[buf][P - 1][ks + lk][wn + lr];
-        const double bi1 = Bs[buf][P - 1][ks + lk][wn + 16 + lr];
+        double ai[BM], bi[BN];
+#pragma unroll
+        for (int mt = 0; mt < BM; ++mt) ai[mt] = As[buf][P - 1][ks + lk][wm + mt * 16 + lr];
+#pragma unroll
+        for (int nt = 0; nt < BN; ++nt) bi[nt] = Bs[buf][P - 1][ks + lk][wn + nt * 16 + lr];
         // re += ar*br - ai*bi ; im += ar*bi + ai*br
-        acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi0, acc_i[0][0], 0, 0, 0);
-        acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi1, acc_i[0][1], 0, 0, 0);
-        acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi0, acc_i[1][0], 0, 0, 0);
-        acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi1, acc_i[1][1], 0, 0, 0);
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai0, bi0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai0, bi1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai1, bi0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai1, bi1, acc[1][1], 0, 0, 0);
-        acc_i[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b0, acc_i[0][0], 0, 0, 0);
-        acc_i[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, b1, acc_i[0][1], 0, 0, 0);
-        acc_i[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b0, acc_i[1][0], 0, 0, 0);
-        acc_i[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, b1, acc_i[1][1], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < BN; ++nt) {
+            acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bi[nt], acc_i[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[mt], bi[nt], acc[mt][nt], 0, 0, 0);
+            acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[mt], b[nt], acc_i[mt][nt], 0, 0, 0);
+          }
       }
     }
     if (has_next) stage(buf ^ 1);
@@ -207,9 +208,9 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   // ---- epilogue: C = alpha * acc + beta * C (C untouched when beta == 0) ----
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int mt = 0; mt < BM; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+    for (int nt = 0; nt < BN; ++nt)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = i0 + wm + mt * 16 + 4 * v + lk;   // f64 16x16x4 output: row 4*reg + lane/16
@@ -390,18 +391,34 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 block(256);
-  const int64_t n0 = plan->cls_tiles[kTileMfma];
   const auto *A = static_cast<const double *>(a);
   const auto *B = static_cast<const double *>(b);
   auto *Cp = static_cast<double *>(c);
-  if (mfma_class && n0 > 0) {
-    if (dtype == QLT_F64)
-      hipLaunchKernelGGL(k_grouped_gemm_f64<false>, dim3(static_cast<unsigned>(n0)), block, 0, st, plan->d_tiles,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
-    else
-      hipLaunchKernelGGL(k_grouped_gemm_f64<true>, dim3(static_cast<unsigned>(n0)), block, 0, st, plan->d_tiles,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+  // MFMA classes: 64x64, 32x64, 64x32, 32x32 (tiles stored class-major)
+  int64_t base = 0;
+  for (int cls = 0; cls < kNumMfmaClasses; ++cls) {
+    const int64_t cnt = plan->cls_tiles[cls];
+    const Tile *tp = plan->d_tiles + base;
+    base += cnt;
+    if (!mfma_class || cnt == 0) continue;
+    const dim3 grid(static_cast<unsigned>(cnt));
+#define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
+  hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, st, tp, plan->d_segs,       \
+                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi)
+    if (dtype == QLT_F64) {
+      if (cls == 0) QLT_LAUNCH_MFMA(false, 64, 64);
+      else if (cls == 1) QLT_LAUNCH_MFMA(false, 32, 64);
+      else if (cls == 2) QLT_LAUNCH_MFMA(false, 64, 32);
+      else QLT_LAUNCH_MFMA(false, 32, 32);
+    } else {
+      if (cls == 0) QLT_LAUNCH_MFMA(true, 64, 64);
+      else if (cls == 1) QLT_LAUNCH_MFMA(true, 32, 64);
+      else if (cls == 2) QLT_LAUNCH_MFMA(true, 64, 32);
+      else QLT_LAUNCH_MFMA(true, 32, 32);
+    }
+#undef QLT_LAUNCH_MFMA
   }
+  const int64_t n0 = base;
   const int64_t n1 = plan->cls_tiles[kTileRows4], n2 = plan->cls_tiles[kTileRows16];
   if (n1 > 0) {
     const Tile *t1 = plan->d_tiles + n0;

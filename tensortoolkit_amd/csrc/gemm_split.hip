@@ -464,10 +464,11 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 struct SplitLayout {
   std::vector<PackMeta> meta_a, meta_b;
   std::vector<PackItem> items;
+  std::vector<Tile> tiles;        // 64 x 64 output tiles of the MFMA-class blocks, longest first
   int64_t plane_elems = 0;
   int64_t exp_elems = 0;
   size_t off_meta_a = 0, off_meta_b = 0, off_items = 0, off_aoffs = 0, off_boffs = 0, off_exps = 0,
-         off_planes = 0, total = 0;
+         off_planes = 0, off_tiles = 0, total = 0;
 };
 
 SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
@@ -478,7 +479,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
   std::vector<char> used_a(plan->n_a_blks, 0), used_b(plan->n_b_blks, 0);
   for (size_t o = 0; o < plan->h_segs.size(); ++o) {
     const Seg &s = plan->h_segs[o];
-    if (s.cls != kTileMfma) continue;   // row-class segments always run natively
+    if (!is_mfma_class(s.cls)) continue;   // row-class segments always run natively
     for (int32_t t = s.t0; t < s.t1; ++t) {
       const ExecTask &e = plan->h_exec[t];
       if (!used_a[e.a_blk]) {
@@ -513,6 +514,21 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
   };
   assign(L.meta_a, used_a, 0);
   assign(L.meta_b, used_b, 1);
+  {
+    std::vector<std::pair<int64_t, int32_t>> order;   // (-work, segment)
+    for (size_t o = 0; o < plan->h_segs.size(); ++o)
+      if (is_mfma_class(plan->h_segs[o].cls)) order.push_back({-plan->h_segs[o].k_total, static_cast<int32_t>(o)});
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int64_t, int32_t> &x, const std::pair<int64_t, int32_t> &y) {
+                       return x.first < y.first;
+                     });
+    for (const auto &e : order) {
+      const Seg &sg = plan->h_segs[e.second];
+      const int ntm = (sg.m + TM - 1) / TM, ntn = (sg.n + TN - 1) / TN;
+      for (int a = 0; a < ntm; ++a)
+        for (int b = 0; b < ntn; ++b) L.tiles.push_back(Tile{e.second, a, b, 0});
+    }
+  }
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };
   size_t off = 0;
   L.off_meta_a = off;
@@ -529,6 +545,8 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
   off = align(off + sizeof(int32_t) * L.exp_elems);
   L.off_planes = off;
   off = align(off + sizeof(uint16_t) * L.plane_elems);
+  L.off_tiles = off;
+  off = align(off + sizeof(Tile) * L.tiles.size());
   L.total = off;
   return L;
 }
@@ -554,12 +572,14 @@ static double g_split_min_mean_k = 48.0;
 constexpr double kSplitMinFlopsPerByte = 96.0;
 
 bool split_profitable(const qlt_plan *plan, int32_t dtype) {
-  if (plan->cls_tiles[kTileMfma] == 0) return false;
+  int64_t mfma_tiles = 0;
+  for (int c = 0; c < kNumMfmaClasses; ++c) mfma_tiles += plan->cls_tiles[c];
+  if (mfma_tiles == 0) return false;
   if (g_split_min_mean_k <= 0.0) return true;   // forced
   double w = 0.0, wk = 0.0, elems = 0.0;
   std::vector<char> seen_a(plan->n_a_blks, 0), seen_b(plan->n_b_blks, 0);
   for (const Seg &s : plan->h_segs) {
-    if (s.cls != kTileMfma) continue;
+    if (!is_mfma_class(s.cls)) continue;
     for (int32_t t = s.t0; t < s.t1; ++t) {
       const ExecTask &e = plan->h_exec[t];
       const double k = e.k;
@@ -610,10 +630,12 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
                     hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_aoffs, aoffs.data(), sizeof(int64_t) * aoffs.size(), hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_boffs, boffs.data(), sizeof(int64_t) * boffs.size(), hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_tiles, L.tiles.data(), sizeof(Tile) * L.tiles.size(), hipMemcpyHostToDevice));
   plan->split_dtype = dtype;
   plan->split_items = static_cast<int64_t>(L.items.size());
-  const size_t offs[7] = {L.off_meta_a, L.off_meta_b, L.off_items, L.off_aoffs, L.off_boffs, L.off_exps,
-                          L.off_planes};
+  plan->split_tiles = static_cast<int64_t>(L.tiles.size());
+  const size_t offs[8] = {L.off_meta_a, L.off_meta_b, L.off_items, L.off_aoffs, L.off_boffs, L.off_exps,
+                          L.off_planes, L.off_tiles};
   std::memcpy(plan->split_off, offs, sizeof(offs));
   return QLT_OK;
 }
@@ -633,8 +655,9 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   const auto *boffs = reinterpret_cast<const int64_t *>(base + so[4]);
   auto *exps = reinterpret_cast<int32_t *>(base + so[5]);
   auto *planes = reinterpret_cast<uint16_t *>(base + so[6]);
+  const auto *stiles = reinterpret_cast<const Tile *>(base + so[7]);
   const unsigned n_items = static_cast<unsigned>(plan->split_items);
-  const unsigned n_mfma = static_cast<unsigned>(plan->cls_tiles[kTileMfma]);
+  const unsigned n_mfma = static_cast<unsigned>(plan->split_tiles);
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 blk(256);
@@ -643,14 +666,14 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
     hipLaunchKernelGGL(k_split_gemm<false>, dim3(n_mfma), blk, 0, st,
-                       plan->d_tiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
+                       stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
                        static_cast<double *>(c), ar, ai, br, bi);
   } else {
     hipLaunchKernelGGL(k_pack<true>, dim3(n_items), blk, 0, st, items, meta_a,
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
     hipLaunchKernelGGL(k_split_gemm<true>, dim3(n_mfma), blk, 0, st,
-                       plan->d_tiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
+                       stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
                        static_cast<double *>(c), ar, ai, br, bi);
   }
   QLT_HIP(hipGetLastError());

@@ -325,12 +325,10 @@ __global__ void k_segs(const int64_t *c_first, const int64_t *c_off, const int32
   int64_t kt = 0;
   for (int32_t t = s.t0; t < s.t1; ++t) kt += exec[t].k;
   s.k_total = kt;
-  s.cls = tile_class_of(n);
+  s.cls = tile_class_of(m, n);
   s.pad = 0;
   segs[o] = s;
-  tile_cnt[o] = (s.cls != kTileMfma)
-                    ? (m + rows_per_tile(s.cls) - 1) / rows_per_tile(s.cls)
-                    : ((m + args.tile_m - 1) / args.tile_m) * ((n + args.tile_n - 1) / args.tile_n);
+  tile_cnt[o] = tiles_of(s.cls, m, n);
   cost[o] = kt;
 }
 
@@ -340,7 +338,7 @@ __global__ void k_tiles(const Seg *segs, const int64_t *tile_start, const int64_
   const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (o >= n_c) return;
   const Seg s = segs[o];
-  const int ntn = (s.cls != kTileMfma) ? 1 : (s.n + args.tile_n - 1) / args.tile_n;
+  const int ntn = is_mfma_class(s.cls) ? (s.n + mfma_tile_n(s.cls) - 1) / mfma_tile_n(s.cls) : 1;
   const int64_t base = tile_start[o];
   for (int64_t q = 0; q < tile_cnt[o]; ++q) {
     Tile t;
@@ -780,13 +778,13 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   int64_t kmax = 0;
   for (const Seg &s : plan->h_segs) {
     kmax = std::max(kmax, s.k_total);
-    const int64_t cnt = (s.cls != kTileMfma)
-                            ? (s.m + rows_per_tile(s.cls) - 1) / rows_per_tile(s.cls)
-                            : ((s.m + args.tile_m - 1) / args.tile_m) * ((s.n + args.tile_n - 1) / args.tile_n);
-    plan->cls_tiles[s.cls] += cnt;
+    plan->cls_tiles[s.cls] += tiles_of(s.cls, s.m, s.n);
   }
-  QLT_CHECK_ARG(plan->cls_tiles[0] + plan->cls_tiles[1] + plan->cls_tiles[2] == n_tiles,
-                "qlt_plan_build: tile count mismatch");
+  {
+    int64_t total = 0;
+    for (int c = 0; c < kNumTileClasses; ++c) total += plan->cls_tiles[c];
+    QLT_CHECK_ARG(total == n_tiles, "qlt_plan_build: tile count mismatch");
+  }
   Tile *d_tiles_raw;
   uint64_t *d_tk0, *d_tk1;
   int32_t *d_tv0, *d_tv1;

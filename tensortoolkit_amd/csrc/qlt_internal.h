@@ -91,19 +91,35 @@ struct Seg {
   int32_t pad;
 };
 
-// Tile classes.  kTileMfma: 64 x 64 output tiles on the MFMA kernels.
+// Tile classes.  Classes 0..3: MFMA output tiles of 64x64, 32x64, 64x32 and
+// 32x32 (rows x cols), chosen per output block to limit padding.
 // kTileRows4 / kTileRows16: narrow outputs (n <= 4 / n <= 16) on a
 // row-streaming kernel (bandwidth-bound work), 1024 / 256 output rows per
 // workgroup (4 / 1 rows per thread).
-constexpr int32_t kTileMfma = 0;
-constexpr int32_t kTileRows4 = 1;
-constexpr int32_t kTileRows16 = 2;
-constexpr int32_t kNumTileClasses = 3;
+constexpr int32_t kTileMfma = 0;       // 64 x 64 (first MFMA class)
+constexpr int32_t kNumMfmaClasses = 4;
+constexpr int32_t kTileRows4 = 4;
+constexpr int32_t kTileRows16 = 5;
+constexpr int32_t kNumTileClasses = 6;
 
-__host__ __device__ constexpr int32_t tile_class_of(int64_t n) {
-  return n <= 4 ? kTileRows4 : (n <= 16 ? kTileRows16 : kTileMfma);
+__host__ __device__ constexpr int32_t mfma_tile_m(int32_t cls) { return (cls == 1 || cls == 3) ? 32 : 64; }
+__host__ __device__ constexpr int32_t mfma_tile_n(int32_t cls) { return (cls == 2 || cls == 3) ? 32 : 64; }
+__host__ __device__ constexpr bool is_mfma_class(int32_t cls) { return cls < kNumMfmaClasses; }
+
+// 32 along a dimension when it cuts the padded extent by at least 10%.
+__host__ __device__ constexpr bool prefer_32(int64_t x) {
+  return ((x + 31) / 32) * 32 * 10 <= ((x + 63) / 64) * 64 * 9;
+}
+__host__ __device__ constexpr int32_t tile_class_of(int64_t m, int64_t n) {
+  return n <= 4 ? kTileRows4
+                : (n <= 16 ? kTileRows16 : (prefer_32(m) ? 1 : 0) + (prefer_32(n) ? 2 : 0));
 }
 __host__ __device__ constexpr int32_t rows_per_tile(int32_t cls) { return cls == kTileRows4 ? 1024 : 256; }
+__host__ __device__ constexpr int64_t tiles_of(int32_t cls, int64_t m, int64_t n) {
+  return is_mfma_class(cls)
+             ? ((m + mfma_tile_m(cls) - 1) / mfma_tile_m(cls)) * ((n + mfma_tile_n(cls) - 1) / mfma_tile_n(cls))
+             : (m + rows_per_tile(cls) - 1) / rows_per_tile(cls);
+}
 
 // One output tile of the grouped kernel.
 struct Tile {
@@ -159,13 +175,14 @@ struct qlt_plan {
   // tile-size class table chosen at build time
   int32_t tile_m = 64;
   int32_t tile_n = 64;
-  int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0};  // tiles per class, class-major in d_tiles
+  int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};  // tiles per class, class-major in d_tiles
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;
   int32_t split_dtype = -1;             // dtype the split metadata was written for
   int64_t split_items = 0;              // pack work items
-  size_t split_off[7] = {0, 0, 0, 0, 0, 0, 0};  // workspace sub-buffer offsets
+  size_t split_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // workspace sub-buffer offsets
+  int64_t split_tiles = 0;              // 64 x 64 tiles of the split kernel
 
   qlt_plan() = default;
   qlt_plan(const qlt_plan &) = delete;
