@@ -8,10 +8,11 @@ inputs resident in HBM.  Throughput = algorithmic flops (sum over tasks of
 8*m*k*n, tensor_op_cost.h:384-406) / time.
 
 The JSON line reports the split-BF16 path (``value``) next to the native
-FP64 path, the per-launch roofline of the dominant grouped-GEMM launch
-(HIP events on the launch stream), the error of the step's output against
-the FP64 CPU oracle, and the CPU baseline: the oracle (the reference
-algorithm restated in C, OpenBLAS zgemm per task) on a bounded sample.
+FP64 path; the roofline of the dominant kernel (the library's kernel probe
+records HIP events around that one kernel launch, on its launch stream,
+in every timed step); the error of the step's output against the FP64 CPU
+oracle; and the CPU baseline: the oracle (the reference algorithm restated
+in C, OpenBLAS zgemm per task) on a bounded sample.
 """
 from __future__ import annotations
 
@@ -99,30 +100,44 @@ class MatvecRunner:
         return self.outs[-1].to_numpy(out.np_dtype, out.raw_size)
 
 
-def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int):
+def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int, probe_step: int):
+    """Times `steps` matvecs.  Inside every step the library's kernel probe
+    brackets the dominant kernel of contraction `probe_step` with its own
+    event pair, recorded on the launch stream."""
+    lib = capi.lib()
     for p in runner.plans:
         p.prepare(runner.dtype, mode)
     for _ in range(warmup):
         for s in range(4):
             runner.launch(s, mode)
-    capi.check(capi.lib().qlt_stream_sync(runner.stream))
+    capi.check(lib.qlt_stream_sync(runner.stream))
     ev = [[(hip.event(), hip.event()) for _ in range(4)] for _ in range(steps)]
+    kev = [(hip.event(), hip.event()) for _ in range(steps)]
     t_start, t_end = hip.event(), hip.event()
-    capi.check(capi.lib().qlt_device_sync())
+    name = C.create_string_buffer(96)
+    kflops, kbytes = C.c_double(), C.c_double()
+    capi.check(lib.qlt_device_sync())
     wall0 = time.perf_counter()
     hip.record(t_start, runner.stream)
     for k in range(steps):
         for s in range(4):
             hip.record(ev[k][s][0], runner.stream)
+            if s == probe_step:
+                capi.check(lib.qlt_kernel_probe_set(*kev[k]))
             runner.launch(s, mode)
+            if s == probe_step:
+                capi.check(lib.qlt_kernel_probe_last(name, 96, C.byref(kflops), C.byref(kbytes)))
+                capi.check(lib.qlt_kernel_probe_set(None, None))
             hip.record(ev[k][s][1], runner.stream)
     hip.record(t_end, runner.stream)
-    capi.check(capi.lib().qlt_stream_sync(runner.stream))
-    capi.check(capi.lib().qlt_device_sync())
+    capi.check(lib.qlt_stream_sync(runner.stream))
+    capi.check(lib.qlt_device_sync())
     wall = time.perf_counter() - wall0
     total_ms = hip.elapsed_ms(t_start, t_end)
     per_launch = np.array([[hip.elapsed_ms(*ev[k][s]) for s in range(4)] for k in range(steps)])
-    return total_ms, wall, per_launch.mean(axis=0)
+    kernel = {"name": name.value.decode(), "flops": kflops.value, "bytes": kbytes.value,
+              "ms": float(np.mean([hip.elapsed_ms(*kev[k]) for k in range(steps)]))}
+    return total_ms, wall, per_launch.mean(axis=0), kernel
 
 
 def time_graph(runner: MatvecRunner, hip: Hip, mode: int, replays: int):
@@ -144,7 +159,65 @@ def time_graph(runner: MatvecRunner, hip: Hip, mode: int, replays: int):
     return ms / replays
 
 
-def cpu_baseline(mv: W.Matvec, budget_s: float = 20.0):
+# Slice products per algorithmic product in the split-BF16 GEMM: five BF16
+# slices per operand, every slice pair (i, j) with i + j <= 4.
+SPLIT_PRODUCTS = 15
+
+
+def profile_traffic(kernel_name: str, workload: str):
+    """HBM bytes per dispatch of `kernel_name` on `workload`, read from the
+    committed rocprofv3 PMC summary (profiles/*.pmc.txt: FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by a device-to-device copy of known
+    size), or None when no summary covers this kernel and workload."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith(".pmc.txt"):
+            continue
+        kv = {}
+        with open(os.path.join(d, fn)) as f:
+            for line in f:
+                if ":" in line and not line.startswith(" "):
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("kernel") == kernel_name and kv.get("workload") == workload and "traffic_bytes_per_dispatch" in kv:
+            return float(kv["traffic_bytes_per_dispatch"]), f"profiles/{fn}"
+    return None, None
+
+
+def kernel_roofline(kernel: dict, workload: str) -> dict:
+    """Roofline of the probed kernel: algorithmic flops (sum of c*m*k*n over
+    its tasks) over its event-timed duration.  The split GEMM runs on BF16
+    MFMA, so its rate is given in BF16 flops (15 slice products per
+    algorithmic product) against the BF16 peak, with the FP64-equivalent
+    rate beside it."""
+    secs = kernel["ms"] * 1e-3
+    fp64_tf = kernel["flops"] / secs / 1e12
+    split = kernel["name"].startswith("k_split_gemm")
+    rows = kernel["name"].startswith("k_rows")
+    traffic, src = profile_traffic(kernel["name"], workload)
+    out = {"kernel": kernel["name"], "ms_per_launch": round(kernel["ms"], 5),
+           "algorithmic_flops": kernel["flops"], "algorithmic_bytes": kernel["bytes"]}
+    if rows:
+        gbs = kernel["bytes"] / secs / 1e9
+        out.update(bound="hbm", achieved=round(gbs, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                   frac=round(gbs / PEAK_HBM_GBS, 4))
+    elif split:
+        bf16_tf = SPLIT_PRODUCTS * fp64_tf
+        out.update(bound="mfma", achieved=round(bf16_tf, 2), peak=PEAK_BF16_TFLOPS, unit="TFLOP/s",
+                   frac=round(bf16_tf / PEAK_BF16_TFLOPS, 4), fp64_equivalent_tflops=round(fp64_tf, 3),
+                   fp64_equivalent_frac_of_fp64_peak=round(fp64_tf / PEAK_FP64_TFLOPS, 4))
+    else:
+        out.update(bound="mfma", achieved=round(fp64_tf, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                   frac=round(fp64_tf / PEAK_FP64_TFLOPS, 4))
+    out["traffic"] = traffic
+    if src:
+        out["traffic_source"] = src
+    return out
+
+
+def cpu_baseline(mv: W.Matvec, budget_s: float = 10.0):
     """The oracle (reference algorithm restated in C, OpenBLAS per task) on a
     bounded sample: whole matvec steps until about `budget_s` seconds."""
     from oracle import oracle as O
@@ -181,7 +254,7 @@ def cpu_baseline(mv: W.Matvec, budget_s: float = 20.0):
         t_exec += time.perf_counter() - t0
         done += 1
         result = x
-        if t_exec >= budget_s or done >= 3:
+        if t_exec >= budget_s or done >= 50:
             break
     return {
         "value": flops_step * done / t_exec / 1e9,
@@ -200,7 +273,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph-replays", type=int, default=20)
+    ap.add_argument("--graph-replays", type=int, default=100)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (scales rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -235,17 +308,20 @@ def main() -> int:
     results = {}
     modes = [("native", capi.QLT_MODE_NATIVE), ("split_bf16", capi.QLT_MODE_SPLIT_BF16)]
     outputs = {}
+    dom = int(np.argmax(runner.flops))
     for name, mode in modes:
         try:
-            total_ms, wall, per_launch = time_mode(runner, hip, mode, args.steps, args.warmup)
+            total_ms, wall, per_launch, kernel = time_mode(runner, hip, mode, args.steps, args.warmup, dom)
         except capi.QltError as e:
             print(f"[bench] mode {name} unavailable: {e}", file=sys.stderr, flush=True)
             continue
         outputs[name] = runner.result()
         graph_ms = time_graph(runner, hip, mode, args.graph_replays) if args.graph_replays else None
-        results[name] = dict(total_ms=total_ms, wall=wall, per_launch_ms=per_launch, graph_ms=graph_ms)
-        print(f"[bench] {name}: {total_ms / args.steps:.3f} ms/step, per launch {np.round(per_launch, 3).tolist()}"
-              f", graph replay {graph_ms:.3f} ms/step" if graph_ms else "", file=sys.stderr, flush=True)
+        results[name] = dict(total_ms=total_ms, wall=wall, per_launch_ms=per_launch, graph_ms=graph_ms,
+                             kernel=kernel)
+        print(f"[bench] {name}: {total_ms / args.steps:.3f} ms/step, per launch {np.round(per_launch, 3).tolist()}, "
+              f"dominant kernel {kernel['name']} {kernel['ms']:.4f} ms"
+              + (f", graph replay {graph_ms:.3f} ms/step" if graph_ms else ""), file=sys.stderr, flush=True)
 
     # correctness of the step output against the FP64 CPU oracle
     errors = {}
@@ -268,9 +344,7 @@ def main() -> int:
     r = results[primary]
     ms_step = r["total_ms"] / args.steps
     value = flops_step / (ms_step * 1e-3) / 1e9
-    dom = int(np.argmax(runner.flops))
-    dom_ms = float(r["per_launch_ms"][dom])
-    achieved_tf = runner.flops[dom] / (dom_ms * 1e-3) / 1e12
+    roofline = kernel_roofline(r["kernel"], mv.name)
     native_gf = None
     if "native" in results:
         native_gf = flops_step / (results["native"]["total_ms"] / args.steps * 1e-3) / 1e9
@@ -290,11 +364,10 @@ def main() -> int:
         "config": {"workload": mv.name, "path": primary, "contractions_per_step": 4,
                    "gflop_per_step": round(flops_step / 1e9, 3),
                    "tasks": [int(p.info.n_tasks) for p in runner.plans]},
-        "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 3),
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": f"grouped GEMM launch of contraction {dom} ({primary})"},
+        "roofline": roofline,
         "native_gflops": round(native_gf, 2) if native_gf else None,
+        "native_roofline": kernel_roofline(results["native"]["kernel"], mv.name) if "native" in results else None,
+        "gflops_plan_plus_execute": round(flops_step / (runner.plan_seconds + ms_step * 1e-3) / 1e9, 2),
         "split_vs_native": round(value / native_gf, 3) if (native_gf and primary != "native") else None,
         "max_rel_err_vs_oracle": {k: float(f"{v:.3e}") for k, v in errors.items()},
         "graph_replay_ms_per_step": {k: round(v["graph_ms"], 4) for k, v in results.items() if v["graph_ms"]},

@@ -182,6 +182,19 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
 qlt_status qlt_set_split_min_mean_k(double min_mean_k);
 double qlt_get_split_min_mean_k(void);
 
+/* Kernel probe, for measurement tools; no counterpart in the reference.
+ * With both events non-null (hipEvent_t created on the current device),
+ * every later qlt_contract_execute outside graph capture records
+ * `start_event` on its stream just before the launch of its dominant kernel
+ * (the tile-class kernel, or the split GEMM, with the most algorithmic
+ * flops) and `stop_event` just after it.  qlt_kernel_probe_last returns that
+ * kernel's name, its algorithmic flops (sum of c*m*k*n, c = 2 real or 8
+ * complex) and its compulsory bytes (distinct A and B blocks it reads plus
+ * the C elements it writes).  Null events disarm the probe.  Process-wide;
+ * meant for one measuring thread. */
+qlt_status qlt_kernel_probe_set(void *start_event, void *stop_event);
+qlt_status qlt_kernel_probe_last(char *name, int32_t name_len, double *flops, double *bytes);
+
 /* ---- graph capture of a repeated contraction chain (Lanczos / DMRG sweep) ---- */
 qlt_status qlt_graph_begin(void *stream);
 qlt_status qlt_graph_end(void *stream, qlt_graph **graph);

@@ -100,6 +100,8 @@ _SIGS = {
                               _f64p, _f64p, C.c_void_p], C.c_int32),
     "qlt_set_split_min_mean_k": ([C.c_double], C.c_int32),
     "qlt_get_split_min_mean_k": ([], C.c_double),
+    "qlt_kernel_probe_set": ([C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_kernel_probe_last": ([C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int32),
     "qlt_graph_begin": ([C.c_void_p], C.c_int32),
     "qlt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int32),
     "qlt_graph_replay": ([C.c_void_p, C.c_void_p], C.c_int32),

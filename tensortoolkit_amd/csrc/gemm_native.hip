@@ -15,6 +15,8 @@
 // k-chunk in LDS, with the thread mapping chosen so that the block's
 // innermost memory axis is read contiguously.
 
+#include <cstdio>
+
 #include "qlt_internal.h"
 
 using namespace qlt;
@@ -23,10 +25,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64;
-constexpr int TN = 64;
-constexpr int KC = 16;
-constexpr int LDP = TM + 1;   // padded LDS row (doubles)
+constexpr int KC = 16;   // k-chunk staged through LDS per pipeline step
 
 // Offset of linear index `x` inside a block, decomposed row-major over
 // dims[0..n-1] (last fastest) with the given memory strides.
@@ -387,7 +386,31 @@ namespace qlt {
 // the row-streaming kernel for the second.  `mfma_class` = false skips the
 // MFMA class (the split path runs it on its own kernel).
 qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
-                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class) {
+                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class,
+                                 bool probe) {
+  const bool cplx = dtype == QLT_C128;
+  // the class with the most algorithmic work is the one a kernel probe times
+  int32_t dom = -1;
+  if (probe && probe_armed(st)) {
+    double best = 0.0;
+    for (int32_t cls = mfma_class ? 0 : kNumMfmaClasses; cls < kNumTileClasses; ++cls)
+      if (plan->cls_tiles[cls] > 0 && plan->cls_mkn[cls] > best) {
+        best = plan->cls_mkn[cls];
+        dom = cls;
+      }
+  }
+  auto probe_in = [&](int32_t cls) {
+    if (cls == dom) probe_begin(st);
+  };
+  auto probe_out = [&](int32_t cls) {
+    if (cls != dom) return;
+    static const char *const kNames[kNumTileClasses] = {"k_grouped_gemm_f64<64x64>", "k_grouped_gemm_f64<32x64>",
+                                                        "k_grouped_gemm_f64<64x32>", "k_grouped_gemm_f64<32x32>",
+                                                        "k_rows_f64<n<=4>", "k_rows_f64<n<=16>"};
+    char name[64];
+    std::snprintf(name, sizeof(name), "%s %s", kNames[cls], cplx ? "c128" : "f64");
+    probe_end(st, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], (cplx ? 16.0 : 8.0) * plan->cls_elems[cls]);
+  };
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 block(256);
@@ -405,6 +428,7 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
 #define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
   hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, st, tp, plan->d_segs,       \
                      plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi)
+    probe_in(cls);
     if (dtype == QLT_F64) {
       if (cls == 0) QLT_LAUNCH_MFMA(false, 64, 64);
       else if (cls == 1) QLT_LAUNCH_MFMA(false, 32, 64);
@@ -416,27 +440,32 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
       else if (cls == 2) QLT_LAUNCH_MFMA(true, 64, 32);
       else QLT_LAUNCH_MFMA(true, 32, 32);
     }
+    probe_out(cls);
 #undef QLT_LAUNCH_MFMA
   }
   const int64_t n0 = base;
   const int64_t n1 = plan->cls_tiles[kTileRows4], n2 = plan->cls_tiles[kTileRows16];
   if (n1 > 0) {
     const Tile *t1 = plan->d_tiles + n0;
+    probe_in(kTileRows4);
     if (dtype == QLT_F64)
       hipLaunchKernelGGL((k_rows_f64<false, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
                          plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
     else
       hipLaunchKernelGGL((k_rows_f64<true, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
                          plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+    probe_out(kTileRows4);
   }
   if (n2 > 0) {
     const Tile *t2 = plan->d_tiles + n0 + n1;
+    probe_in(kTileRows16);
     if (dtype == QLT_F64)
       hipLaunchKernelGGL((k_rows_f64<false, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
                          plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
     else
       hipLaunchKernelGGL((k_rows_f64<true, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
                          plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
+    probe_out(kTileRows16);
   }
   QLT_HIP(hipGetLastError());
   return QLT_OK;
@@ -444,7 +473,7 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
 
 qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                          const double *alpha, const double *beta, hipStream_t st) {
-  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, true);
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, true, true);
 }
 
 }  // namespace qlt

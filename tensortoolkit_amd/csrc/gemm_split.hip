@@ -558,7 +558,8 @@ namespace qlt {
 qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                          const double *alpha, const double *beta, hipStream_t st);
 qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
-                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class);
+                                 const double *alpha, const double *beta, hipStream_t st, bool mfma_class,
+                                 bool probe);
 
 // Flop-weighted mean k below which a plan runs natively even in split mode:
 // such contractions are bound by memory traffic, not by the MFMA rate, and
@@ -661,10 +662,16 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 blk(256);
+  // a kernel probe times the split GEMM unless the row classes carry more work
+  double mfma_mkn = 0.0, rows_mkn = 0.0;
+  for (int32_t cls = 0; cls < kNumTileClasses; ++cls) (is_mfma_class(cls) ? mfma_mkn : rows_mkn) += plan->cls_mkn[cls];
+  const bool probe_split = probe_armed(st) && mfma_mkn >= rows_mkn;
+  const bool cplx = dtype == QLT_C128;
   if (dtype == QLT_F64) {
     hipLaunchKernelGGL(k_pack<false>, dim3(n_items), blk, 0, st, items, meta_a,
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+    if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_gemm<false>, dim3(n_mfma), blk, 0, st,
                        stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
                        static_cast<double *>(c), ar, ai, br, bi);
@@ -672,13 +679,20 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
     hipLaunchKernelGGL(k_pack<true>, dim3(n_items), blk, 0, st, items, meta_a,
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+    if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_gemm<true>, dim3(n_mfma), blk, 0, st,
                        stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
                        static_cast<double *>(c), ar, ai, br, bi);
   }
+  if (probe_split) {
+    double elems = 0.0;
+    for (int32_t cls = 0; cls < kNumMfmaClasses; ++cls) elems += plan->cls_elems[cls];
+    probe_end(st, cplx ? "k_split_gemm c128" : "k_split_gemm f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
+              (cplx ? 16.0 : 8.0) * elems);
+  }
   QLT_HIP(hipGetLastError());
   // narrow outputs: the native row-streaming kernel
-  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, false);
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, false, !probe_split);
 }
 
 }  // namespace qlt

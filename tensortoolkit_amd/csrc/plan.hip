@@ -780,6 +780,27 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
     kmax = std::max(kmax, s.k_total);
     plan->cls_tiles[s.cls] += tiles_of(s.cls, s.m, s.n);
   }
+  // Per-class algorithmic work (m*k*n summed) and compulsory elements
+  // (distinct A and B blocks a class reads, plus the C blocks it writes).
+  {
+    std::vector<uint8_t> seen_a(static_cast<size_t>(plan->n_a_blks), 0), seen_b(static_cast<size_t>(plan->n_b_blks), 0);
+    for (const Seg &s : plan->h_segs) {
+      plan->cls_mkn[s.cls] += static_cast<double>(s.m) * static_cast<double>(s.n) * static_cast<double>(s.k_total);
+      plan->cls_elems[s.cls] += static_cast<double>(s.m) * static_cast<double>(s.n);
+      const uint8_t bit = static_cast<uint8_t>(1u << s.cls);
+      for (int32_t t = s.t0; t < s.t1; ++t) {
+        const ExecTask &e = plan->h_exec[static_cast<size_t>(t)];
+        if (!(seen_a[e.a_blk] & bit)) {
+          seen_a[e.a_blk] |= bit;
+          plan->cls_elems[s.cls] += static_cast<double>(s.m) * static_cast<double>(e.k);
+        }
+        if (!(seen_b[e.b_blk] & bit)) {
+          seen_b[e.b_blk] |= bit;
+          plan->cls_elems[s.cls] += static_cast<double>(e.k) * static_cast<double>(s.n);
+        }
+      }
+    }
+  }
   {
     int64_t total = 0;
     for (int c = 0; c < kNumTileClasses; ++c) total += plan->cls_tiles[c];

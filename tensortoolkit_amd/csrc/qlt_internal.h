@@ -137,6 +137,12 @@ struct GatherLayout {
   int32_t b_nfast;     // B's innermost memory axis is its last saved axis
 };
 
+// Kernel probe (qlt_kernel_probe_set): events recorded around the dominant
+// kernel of an execute call.
+bool probe_armed(hipStream_t st);
+void probe_begin(hipStream_t st);
+void probe_end(hipStream_t st, const char *name, double flops, double bytes);
+
 }  // namespace qlt
 
 // Opaque plan (definition shared between plan.hip and the GEMM units).
@@ -176,6 +182,8 @@ struct qlt_plan {
   int32_t tile_m = 64;
   int32_t tile_n = 64;
   int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};  // tiles per class, class-major in d_tiles
+  double cls_mkn[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};     // sum of m*k*n over the class's tasks
+  double cls_elems[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};   // distinct A + B block elements + C elements
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;

@@ -47,6 +47,33 @@ bool stream_capturing(hipStream_t s) {
   return st == hipStreamCaptureStatusActive;
 }
 
+// ---- kernel probe ----
+namespace {
+struct KernelProbe {
+  hipEvent_t start = nullptr;
+  hipEvent_t stop = nullptr;
+  std::string name;
+  double flops = 0.0;
+  double bytes = 0.0;
+  bool fired = false;
+};
+KernelProbe g_probe;
+std::mutex g_probe_mutex;
+}  // namespace
+
+bool probe_armed(hipStream_t st) { return g_probe.start != nullptr && !stream_capturing(st); }
+
+void probe_begin(hipStream_t st) { (void)hipEventRecord(g_probe.start, st); }
+
+void probe_end(hipStream_t st, const char *name, double flops, double bytes) {
+  (void)hipEventRecord(g_probe.stop, st);
+  std::lock_guard<std::mutex> lock(g_probe_mutex);
+  g_probe.name = name;
+  g_probe.flops = flops;
+  g_probe.bytes = bytes;
+  g_probe.fired = true;
+}
+
 }  // namespace qlt
 
 using namespace qlt;
@@ -203,6 +230,27 @@ inline unsigned grid_for(int64_t n, int threads = 256, int64_t cap = 8192) {
 extern "C" {
 
 const char *qlt_last_error(void) { return g_last_error.c_str(); }
+
+qlt_status qlt_kernel_probe_set(void *start_event, void *stop_event) {
+  QLT_CHECK_ARG((start_event == nullptr) == (stop_event == nullptr),
+                "qlt_kernel_probe_set: give both events or neither");
+  std::lock_guard<std::mutex> lock(g_probe_mutex);
+  g_probe.start = static_cast<hipEvent_t>(start_event);
+  g_probe.stop = static_cast<hipEvent_t>(stop_event);
+  g_probe.fired = false;
+  return QLT_OK;
+}
+
+qlt_status qlt_kernel_probe_last(char *name, int32_t name_len, double *flops, double *bytes) {
+  QLT_CHECK_ARG(name != nullptr && name_len > 0 && flops != nullptr && bytes != nullptr,
+                "qlt_kernel_probe_last: null output");
+  std::lock_guard<std::mutex> lock(g_probe_mutex);
+  if (!g_probe.fired) return fail(QLT_ERR_INVALID_ARG, "qlt_kernel_probe_last: no probed launch since arming");
+  std::snprintf(name, static_cast<size_t>(name_len), "%s", g_probe.name.c_str());
+  *flops = g_probe.flops;
+  *bytes = g_probe.bytes;
+  return QLT_OK;
+}
 
 qlt_status qlt_init(int32_t device) {
   std::lock_guard<std::mutex> lock(g_init_mutex);

@@ -173,3 +173,41 @@ def test_small_matvec_chain(qlt, u1u1, dtype):
         assert per_block_rel_err(c, c_ref.data) <= TOL, step
         x, x_ref = c, c_ref
     assert [ix.dir for ix in x.indexes] == [IN, IN, IN, OUT]
+
+
+def test_kernel_probe_names_and_counts_dominant_kernel(qlt):
+    """The probe brackets one kernel and reports its algorithmic flops: the
+    rank-3 U(1) pair has 32 tasks of (16,16,16), all on the n <= 16 row kernel,
+    2 * 32 * 16**3 = 262144 flops (the 2.62e5 of the workload definition)."""
+    import ctypes as C
+    _say("case: kernel probe on the rank-3 U(1) pair")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    lib = capi.lib()
+    a, b, ax = W.config0()
+    capi.check(lib.qlt_kernel_probe_set(e0, e1))
+    try:
+        plan, c = gpu_contract(a, b, *ax)
+        name = C.create_string_buffer(96)
+        fl, by = C.c_double(), C.c_double()
+        capi.check(lib.qlt_kernel_probe_last(name, 96, C.byref(fl), C.byref(by)))
+    finally:
+        capi.check(lib.qlt_kernel_probe_set(None, None))
+    assert hip.hipEventSynchronize(e1) == 0
+    ms = C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+    assert ms.value > 0.0
+    assert name.value.decode() == "k_rows_f64<n<=16> f64"
+    assert fl.value == 262144.0
+    t = plan.tasks()
+    elems = 32 * 256 + 256 * len(set(t["a_blk_idx"])) + 256 * len(set(t["b_blk_idx"]))
+    assert by.value == 8.0 * elems
+    # disarmed: a later call records nothing
+    assert lib.qlt_kernel_probe_last(name, 96, C.byref(fl), C.byref(by)) != 0
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
