@@ -266,6 +266,63 @@ def cpu_baseline(mv: W.Matvec, budget_s: float = 10.0):
     }, result
 
 
+def run_e2e(args) -> int:
+    """End-to-end two-site update on the U(1)xU(1) D=4096 complex128 tensors
+    made Hermitian: 20 Lanczos steps with the matvec in split-BF16 mode and
+    FP64 reorthogonalisation, then the truncated block SVD of the ground
+    state over (l, s1 | s2, r) at cutoff 1e-10, Dmax = 4096.  The same
+    computation in full FP64 on the CPU (oracle matvec, numpy LAPACK SVD) is
+    the reference for the eigenvalue and truncation-weight errors."""
+    from tensortoolkit_amd import decompose as DEC, lanczos as LZ
+    capi.init(0)
+    stream = C.c_void_p()
+    capi.check(capi.lib().qlt_stream_create(C.byref(stream)))
+    t0 = time.perf_counter()
+    mv = W.config4()
+    print(f"[e2e] tensors built in {time.perf_counter() - t0:.1f}s, psi {mv.psi.raw_size} elements",
+          file=sys.stderr, flush=True)
+    steps, cutoff, dmax = args.lanczos_steps, 1e-10, 4096
+    mode = capi.QLT_MODE_SPLIT_BF16 if args.e2e_mode == "split" else capi.QLT_MODE_NATIVE
+    hv = LZ.DeviceMatvec(mv, mode, stream)
+    warm_in = capi.DeviceBuffer.from_numpy(mv.psi.data)
+    warm_out = capi.DeviceBuffer(mv.psi.raw_size * 16)
+    hv.apply(warm_in.ptr, warm_out.ptr)          # warm-up (first-launch costs)
+    capi.check(capi.lib().qlt_stream_sync(stream))
+    lz = LZ.lanczos(hv, mv.psi.data.copy(), steps)
+    print(f"[e2e] gpu lanczos: {lz.seconds:.3f}s ({lz.matvec_seconds:.3f}s in matvecs), eigenvalue {lz.eigenvalue!r}",
+          file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    res = DEC.svd(mv.psi, 2, (0, 0), cutoff, 1, dmax, t_dev=lz.ground_state, stream=stream)
+    svd_s = time.perf_counter() - t1
+    print(f"[e2e] gpu svd: {svd_s:.3f}s, D {res.D}, trunc err {res.trunc_err!r}, sweeps {res.sweeps}",
+          file=sys.stderr, flush=True)
+    line = {"metric": "two-site update: 20-step Lanczos (split-BF16 matvec, FP64 orthogonalisation) + FP64 block SVD",
+            "workload": mv.name, "lanczos_steps": steps, "matvec_mode": args.e2e_mode,
+            "gpu": {"eigenvalue": lz.eigenvalue, "trunc_err": res.trunc_err, "D": res.D,
+                    "lanczos_s": round(lz.seconds, 4), "matvec_s": round(lz.matvec_seconds, 4),
+                    "svd_s": round(svd_s, 4), "svd_sweeps": res.sweeps,
+                    "svd_block_matrices": len(res.svals)}}
+    if not args.no_cpu_baseline:
+        from oracle import decomp as OD, lanczos as OL
+        from oracle import oracle as O_
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores) or cores))
+        O_.use_openblas(cores)
+        theta, gs, _, _, cpu_lz_s = OL.lanczos(mv, mv.psi.data.copy(), steps)
+        psi = mv.psi
+        psi_cpu = type(psi)(psi.indexes, psi.coords, psi.dims, dtype=psi.dtype, data=gs, offsets=psi.offsets)
+        t2 = time.perf_counter()
+        _, err_cpu, D_cpu, _ = OD.svd(psi_cpu, 2, cutoff, 1, dmax)
+        cpu_svd_s = time.perf_counter() - t2
+        line["cpu_fp64"] = {"eigenvalue": theta, "trunc_err": err_cpu, "D": D_cpu,
+                            "lanczos_s": round(cpu_lz_s, 3), "svd_s": round(cpu_svd_s, 3), "cores": cores}
+        line["eigenvalue_rel_err"] = abs(lz.eigenvalue - theta) / abs(theta)
+        line["trunc_err_abs_diff"] = abs(res.trunc_err - err_cpu)
+        line["D_equal"] = res.D == D_cpu
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,12 +334,18 @@ def main() -> int:
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (scales rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--e2e", action="store_true",
+                    help="run the end-to-end two-site update (Lanczos + block SVD) instead of the Contract benchmark")
+    ap.add_argument("--e2e-mode", default="split", choices=["split", "native"])
+    ap.add_argument("--lanczos-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the contraction would shard by output block, see DESIGN.md)", file=sys.stderr)
         return 2
 
+    if args.e2e:
+        return run_e2e(args)
     capi.init(0)
     if args.calibration_copy_mb:
         nbytes = args.calibration_copy_mb << 20

@@ -11,5 +11,7 @@
 #include "qlten/tensor_manipulation/basic_operations.h"
 #include "qlten/tensor_manipulation/contract_contiguous_axes.h"
 #include "qlten/tensor_manipulation/ten_ctrct.h"
+#include "qlten/tensor_manipulation/ten_decomp/ten_qr.h"
+#include "qlten/tensor_manipulation/ten_decomp/ten_svd.h"
 
 #endif

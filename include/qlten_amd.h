@@ -44,7 +44,8 @@ enum {
   QLT_ERR_OUT_OF_MEMORY = 4,
   QLT_ERR_UNSUPPORTED = 5,
   QLT_ERR_CAPTURE = 6,
-  QLT_ERR_INTERNAL = 7
+  QLT_ERR_INTERNAL = 7,
+  QLT_ERR_NOT_CONVERGED = 8
 };
 
 /* Element types (reference QLTEN_Double / QLTEN_Complex, framework/value_t.h:37-81). */
@@ -225,21 +226,48 @@ qlt_status qlt_conjugate(int64_t n, double *z, void *stream);
 qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,
                               const int64_t *lengths, const double *s, void *x, void *stream);
 
-/* ---- batched FP64 block decompositions (replaces MatSVD / MatQR, lapack.h:41-620,637-1916) ----
- * n_mats dense row-major matrices, matrix i has rows[i] x cols[i] doubles at
- * a + a_offsets[i] (device).  SVD: A = U diag(S) Vt with U (rows x r), S (r),
- * Vt (r x cols), r = min(rows, cols), singular values descending, written at
- * u + u_offsets[i], s + s_offsets[i], vt + vt_offsets[i].  QR: A = Q R with
- * Q (rows x r), R (r x cols).  All offsets are host arrays in elements. */
-qlt_status qlt_block_svd_f64(int64_t n_mats, const int64_t *rows, const int64_t *cols,
-                             const double *a, const int64_t *a_offsets,
-                             double *u, const int64_t *u_offsets,
-                             double *s, const int64_t *s_offsets,
-                             double *vt, const int64_t *vt_offsets, void *stream);
-qlt_status qlt_block_qr_f64(int64_t n_mats, const int64_t *rows, const int64_t *cols,
-                            const double *a, const int64_t *a_offsets,
-                            double *q, const int64_t *q_offsets,
-                            double *r, const int64_t *r_offsets, void *stream);
+/* Batched 2-D copy: rectangle i (rows[i] x cols[i] elements of dtype) from
+ * src + src_offsets[i] with row stride src_ld[i] to dst + dst_offsets[i] with
+ * row stride dst_ld[i]; host arrays, counted in elements.  Assembles block
+ * matrices from blocks and scatters decomposition results back, as
+ * RawDataGenDenseDataBlkMat_ / SubMatMemCpy (raw_data_operations.h:550-575)
+ * and DataBlkCopySVDUdata / DataBlkCopySVDVtData / DataBlkCopyQRQdata
+ * (data_blk_operations.h:953-1078) do per block.  Synchronous. */
+qlt_status qlt_copy_blocks_2d(int32_t dtype, int64_t n, const int64_t *rows, const int64_t *cols,
+                              const void *src, const int64_t *src_offsets, const int64_t *src_ld,
+                              void *dst, const int64_t *dst_offsets, const int64_t *dst_ld, void *stream);
+
+/* ---- batched FP64 / complex128 block decompositions ----
+ * Replace MatSVD (framework/hp_numeric/lapack.h:41-82 gesvd; GPU :811-1011)
+ * and MatQR (lapack.h:332-487 geqrf + orgqr/ungqr), which the reference calls
+ * once per block matrix from DataBlkDecompSVD / DataBlkDecompQR
+ * (qltensor/blk_spar_data_ten/data_blk_operations.h:630-663,1028-1046).
+ * n_mats dense row-major matrices of `dtype`; matrix i is rows[i] x cols[i]
+ * at a + a_offsets[i].  Offsets are host arrays counted in elements of dtype
+ * (complex elements for QLT_C128); s and its offsets are in doubles.  Both
+ * calls are synchronous: they return after the outputs are written.  They
+ * allocate device workspace and cannot be graph-captured.
+ *
+ * SVD: A = U diag(S) Vt with r = min(rows, cols), U rows x r, S r (descending),
+ * Vt r x cols, all row-major, by one-sided block Jacobi.  For sigma_k == 0 the
+ * k-th column of U (rows >= cols) or row of Vt (rows < cols) is zero rather
+ * than an orthonormal completion.  `sweeps` (nullable) receives the Jacobi
+ * sweeps run; QLT_ERR_NOT_CONVERGED if 40 sweeps did not converge (outputs
+ * are still written).  min(rows, cols) <= 4096 per matrix.
+ *
+ * QR: A = Q R with k = min(rows, cols), Q rows x k with orthonormal columns,
+ * R k x cols upper triangular, by Householder reflections with LAPACK's
+ * reflector convention (the diagonal of R has geqrf's signs). */
+qlt_status qlt_block_svd(int32_t dtype, int64_t n_mats, const int64_t *rows, const int64_t *cols,
+                         const void *a, const int64_t *a_offsets,
+                         void *u, const int64_t *u_offsets,
+                         double *s, const int64_t *s_offsets,
+                         void *vt, const int64_t *vt_offsets,
+                         int32_t *sweeps, void *stream);
+qlt_status qlt_block_qr(int32_t dtype, int64_t n_mats, const int64_t *rows, const int64_t *cols,
+                        const void *a, const int64_t *a_offsets,
+                        void *q, const int64_t *q_offsets,
+                        void *r, const int64_t *r_offsets, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,17 +1,723 @@
-// Batched FP64 block SVD / QR.
+// Batched dense decompositions of the block matrices of a block-sparse
+// tensor, in FP64 and complex128: SVD by one-sided block Jacobi, QR by
+// Householder reflections.  They replace MatSVD (framework/hp_numeric/
+// lapack.h:41-82 gesvd, GPU :811-1011) and MatQR (lapack.h:332-487 geqrf +
+// orgqr/ungqr) called once per block matrix by DataBlkDecompSVD /
+// DataBlkDecompQR (qltensor/blk_spar_data_ten/data_blk_operations.h:630-663,
+// 1028-1046); here every block matrix of the tensor is decomposed in one
+// batched launch sequence.
+//
+// SVD.  The working matrix X is A (rows >= cols) or A^H (rows < cols), stored
+// column-major so that columns are contiguous; V starts as the identity.  The
+// columns are split into blocks of kB; every round of a sweep pairs the
+// blocks by the circle method and one workgroup per block pair
+//   1. forms the Gram matrix G = P^H P of the panel P = [X_I X_J],
+//   2. skips the pair when max |G_pq| / sqrt(G_pp G_qq) < tol,
+//   3. diagonalises G in LDS by cyclic two-sided Jacobi, accumulating the
+//      unitary Q (G <- J^H G J, Q <- Q J per rotation),
+//   4. updates P <- P Q and the matching columns of V <- V Q.
+// Before every sweep the columns are ordered by decreasing norm (the block
+// order follows it), which keeps graded spectra to a few sweeps.  A matrix
+// whose sweep rotates nothing has converged; then sigma_j = |X_j|,
+// U_j = X_j / sigma_j (zero when sigma_j = 0) and A = U diag(sigma) V^H,
+// sorted by decreasing sigma (ties by column).
+//
+// QR.  One workgroup per matrix runs the unblocked Householder QR with
+// LAPACK's reflector convention (xLARFG: beta = -sign(Re alpha) * |(alpha, x)|,
+// tau = (beta - alpha) / beta, v = x / (alpha - beta); no reflection when x = 0
+// and alpha is real), so R has the same diagonal signs as geqrf, then forms
+// Q = H_0 ... H_{k-1} I[:, :k] as orgqr / ungqr do.
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "qlt_internal.h"
+
+namespace {
+
+struct cplx {
+  double x, y;
+};
+
+__device__ __forceinline__ double re_(double a) { return a; }
+__device__ __forceinline__ double re_(cplx a) { return a.x; }
+__device__ __forceinline__ double conj_(double a) { return a; }
+__device__ __forceinline__ cplx conj_(cplx a) { return {a.x, -a.y}; }
+__device__ __forceinline__ double mul_(double a, double b) { return a * b; }
+__device__ __forceinline__ cplx mul_(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// conj(a) * b
+__device__ __forceinline__ double cmul_(double a, double b) { return a * b; }
+__device__ __forceinline__ cplx cmul_(cplx a, cplx b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double add_(double a, double b) { return a + b; }
+__device__ __forceinline__ cplx add_(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ double sub_(double a, double b) { return a - b; }
+__device__ __forceinline__ cplx sub_(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ double scl_(double s, double a) { return s * a; }
+__device__ __forceinline__ cplx scl_(double s, cplx a) { return {s * a.x, s * a.y}; }
+__device__ __forceinline__ double abs2_(double a) { return a * a; }
+__device__ __forceinline__ double abs2_(cplx a) { return a.x * a.x + a.y * a.y; }
+template <typename T> __device__ __forceinline__ T zero_();
+template <> __device__ __forceinline__ double zero_<double>() { return 0.0; }
+template <> __device__ __forceinline__ cplx zero_<cplx>() { return {0.0, 0.0}; }
+template <typename T> __device__ __forceinline__ T one_();
+template <> __device__ __forceinline__ double one_<double>() { return 1.0; }
+template <> __device__ __forceinline__ cplx one_<cplx>() { return {1.0, 0.0}; }
+// unit phase of g (|g| > 0), conjugated: e^{-i arg g}
+__device__ __forceinline__ double uphase_conj(double g, double ag) { return g / ag; }
+__device__ __forceinline__ cplx uphase_conj(cplx g, double ag) { return {g.x / ag, -g.y / ag}; }
+// complex reciprocal
+__device__ __forceinline__ double recip_(double a) { return 1.0 / a; }
+__device__ __forceinline__ cplx recip_(cplx a) {
+  const double d = a.x * a.x + a.y * a.y;
+  return {a.x / d, -a.y / d};
+}
+
+template <bool C> struct ElemOf { using T = double; };
+template <> struct ElemOf<true> { using T = cplx; };
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kB = 16;          // columns per Jacobi block
+constexpr int kW = 2 * kB;      // panel width
+constexpr int kRC = 16;         // rows per staged panel chunk
+constexpr int kMaxSweeps = 40;
+constexpr int kInnerSweeps = 30;
+constexpr int kMaxSortCols = 4096;   // LDS bound of the per-matrix column sort
+
+struct SvdMat {
+  int64_t M, N;       // working shape (M >= N)
+  int64_t x_off;      // X: column-major M x N, in elements of the workspace
+  int64_t v_off;      // V: column-major N x N
+  int64_t p_off;      // column order (int32), in int32 elements
+  int64_t m, n;       // caller shape
+  int64_t a_off, u_off, s_off, vt_off;
+  int32_t nb, P;      // column blocks; players of the circle method (even)
+  int32_t tr;         // X = A^H
+  int32_t pad;
+};
+
+struct SvdItem {
+  int32_t mat;
+  int32_t k;          // pair index within a round
+};
+
+struct JRot {
+  int32_t p, q;
+  double c, s;
+};
+
+// Circle-method pairing: `players` even, round r in [0, players - 1).
+__device__ __forceinline__ void circle_pair(int players, int r, int k, int &a, int &b) {
+  if (k == 0) {
+    a = players - 1;
+    b = r;
+  } else {
+    a = (r + k) % (players - 1);
+    b = (r - k + players - 1) % (players - 1);
+  }
+}
+
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_load(const SvdMat *mats, const void *a_, void *ws_, int32_t *perm) {
+  using T = typename ElemOf<C>::T;
+  const SvdMat mt = mats[blockIdx.y];
+  const T *a = static_cast<const T *>(a_) + mt.a_off;
+  T *X = static_cast<T *>(ws_) + mt.x_off;
+  T *V = static_cast<T *>(ws_) + mt.v_off;
+  const int64_t MN = mt.M * mt.N, NN = mt.N * mt.N;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t e = blockIdx.x * kThreads + threadIdx.x; e < MN; e += stride) {
+    const int64_t j = e / mt.M, i = e - j * mt.M;   // X(i, j) column-major
+    X[e] = mt.tr ? conj_(a[e]) : a[i * mt.n + j];
+  }
+  for (int64_t e = blockIdx.x * kThreads + threadIdx.x; e < NN; e += stride) {
+    const int64_t j = e / mt.N, i = e - j * mt.N;
+    V[e] = (i == j) ? one_<T>() : zero_<T>();
+  }
+  for (int64_t e = blockIdx.x * kThreads + threadIdx.x; e < mt.N; e += stride)
+    perm[mt.p_off + e] = static_cast<int32_t>(e);
+}
+
+// Squared column norms of X into sn[0..N), then ord[rank] = column by
+// decreasing norm (ties by column index).
+template <typename T>
+__device__ void norms_and_order(const T *X, int64_t M, int64_t N, double *sn, int32_t *ord) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t c = wave; c < N; c += kWaves) {
+    double acc = 0.0;
+    const T *col = X + c * M;
+    for (int64_t i = lane; i < M; i += 64) acc += abs2_(col[i]);
+    acc = wave_sum(acc);
+    if (lane == 0) sn[c] = acc;
+  }
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < N; c += kThreads) {
+    const double v = sn[c];
+    int32_t rank = 0;
+    for (int64_t j = 0; j < N; ++j) {
+      const double w = sn[j];
+      rank += (w > v) || (w == v && j < c);
+    }
+    ord[rank] = static_cast<int32_t>(c);
+  }
+  __syncthreads();
+}
+
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_sort(const SvdMat *mats, const void *ws_, int32_t *perm,
+                                                        const int32_t *active) {
+  using T = typename ElemOf<C>::T;
+  __shared__ double sn[kMaxSortCols];
+  __shared__ int32_t so[kMaxSortCols];
+  const int mi = blockIdx.x;
+  if (!active[mi]) return;
+  const SvdMat mt = mats[mi];
+  if (mt.N > kMaxSortCols) return;   // keep the previous order
+  const T *X = static_cast<const T *>(ws_) + mt.x_off;
+  const int32_t *old = perm + mt.p_off;
+  // norms in the current physical order; the new order is a permutation of
+  // physical columns, so `old` is not needed beyond its length
+  (void)old;
+  norms_and_order(X, mt.M, mt.N, sn, so);
+  for (int64_t c = threadIdx.x; c < mt.N; c += kThreads) perm[mt.p_off + c] = so[c];
+}
+
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_round(const SvdMat *mats, const SvdItem *items, int32_t round,
+                                                         void *ws_, const int32_t *perm, const int32_t *active,
+                                                         int32_t *rotated) {
+  using T = typename ElemOf<C>::T;
+  __shared__ T sP[kRC][kW + 1];
+  __shared__ T sG[kW][kW + 1];
+  __shared__ T sQ[kW][kW + 1];
+  __shared__ JRot sRot[kW / 2];
+  __shared__ T sEm[kW / 2];
+  __shared__ int32_t sCol[kW];
+  __shared__ double sRed[kWaves];
+  __shared__ int32_t sAny;
+
+  const SvdItem it = items[blockIdx.x];
+  if (!active[it.mat]) return;
+  const SvdMat mt = mats[it.mat];
+  if (round >= mt.P - 1) return;
+  int bi, bj;
+  circle_pair(mt.P, round, it.k, bi, bj);
+  const int tid = threadIdx.x;
+  const int32_t *pm = perm + mt.p_off;
+  const int wi = bi < mt.nb ? static_cast<int>(min<int64_t>(kB, mt.N - int64_t(bi) * kB)) : 0;
+  const int wj = bj < mt.nb ? static_cast<int>(min<int64_t>(kB, mt.N - int64_t(bj) * kB)) : 0;
+  const int w = wi + wj;
+  if (w < 2) return;
+  if (tid < w) sCol[tid] = tid < wi ? pm[int64_t(bi) * kB + tid] : pm[int64_t(bj) * kB + (tid - wi)];
+  T *X = static_cast<T *>(ws_) + mt.x_off;
+  T *V = static_cast<T *>(ws_) + mt.v_off;
+  __syncthreads();
+
+  // ---- 1. Gram matrix of the panel ----
+  T acc[4] = {zero_<T>(), zero_<T>(), zero_<T>(), zero_<T>()};
+  for (int64_t r0 = 0; r0 < mt.M; r0 += kRC) {
+    for (int idx = tid; idx < kRC * w; idx += kThreads) {
+      const int c = idx / kRC, rr = idx - c * kRC;
+      const int64_t row = r0 + rr;
+      sP[rr][c] = row < mt.M ? X[int64_t(sCol[c]) * mt.M + row] : zero_<T>();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + u * kThreads;
+      if (e < w * w) {
+        const int p = e / w, q = e - p * w;
+        T s = acc[u];
+#pragma unroll 4
+        for (int rr = 0; rr < kRC; ++rr) s = add_(s, cmul_(sP[rr][p], sP[rr][q]));
+        acc[u] = s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + u * kThreads;
+    if (e < w * w) sG[e / w][e % w] = acc[u];
+  }
+  __syncthreads();
+
+  // ---- 2. convergence test of the pair ----
+  double off = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + u * kThreads;
+    if (e < w * w) {
+      const int p = e / w, q = e - p * w;
+      if (p < q) {
+        const double dp = re_(sG[p][p]), dq = re_(sG[q][q]);
+        if (dp > 0.0 && dq > 0.0) off = fmax(off, abs2_(sG[p][q]) / (dp * dq));
+      }
+    }
+  }
+  off = wave_max(off);
+  if ((tid & 63) == 0) sRed[tid >> 6] = off;
+  __syncthreads();
+  const double tol = fmax(16.0, static_cast<double>(mt.M)) * 2.220446049250313e-16;
+  const double offmax = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
+  if (offmax < tol * tol) return;
+  if (tid == 0) rotated[it.mat] = 1;
+
+  // ---- 3. Jacobi diagonalisation of G in LDS ----
+  for (int e = tid; e < w * w; e += kThreads) {
+    const int p = e / w, q = e - p * w;
+    sQ[p][q] = (p == q) ? one_<T>() : zero_<T>();
+  }
+  const int P2 = w + (w & 1);
+  const double eps = 2.220446049250313e-16;
+  for (int sweep = 0; sweep < kInnerSweeps; ++sweep) {
+    if (tid == 0) sAny = 0;
+    __syncthreads();
+    for (int r = 0; r < P2 - 1; ++r) {
+      if (tid < P2 / 2) {
+        int p, q;
+        circle_pair(P2, r, tid, p, q);
+        JRot rt{-1, -1, 1.0, 0.0};
+        T em = one_<T>();
+        if (p < w && q < w) {
+          const double al = re_(sG[p][p]), be = re_(sG[q][q]);
+          const T g = sG[p][q];
+          const double ag = sqrt(abs2_(g));
+          if (ag > 0.0 && al > 0.0 && be > 0.0 && ag > eps * sqrt(al * be)) {
+            const double zeta = (be - al) / (2.0 * ag);
+            double t;
+            if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+            else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            const double c = 1.0 / sqrt(1.0 + t * t);
+            rt = JRot{p, q, c, c * t};
+            em = uphase_conj(g, ag);
+            sAny = 1;
+          }
+        }
+        sRot[tid] = rt;
+        sEm[tid] = em;
+      }
+      __syncthreads();
+      // columns of G and Q:  [x_p, x_q] <- [x_p, x_q] J,
+      // J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
+      for (int idx = tid; idx < (P2 / 2) * w * 2; idx += kThreads) {
+        const int which = idx / ((P2 / 2) * w);
+        const int rem = idx - which * (P2 / 2) * w;
+        const int k = rem / w, i = rem - k * w;
+        const JRot rt = sRot[k];
+        if (rt.p < 0) continue;
+        const T em = sEm[k];
+        T(*Mx)[kW + 1] = which ? sQ : sG;
+        const T xp = Mx[i][rt.p], xq = Mx[i][rt.q];
+        const T emxq = mul_(em, xq);
+        Mx[i][rt.p] = sub_(scl_(rt.c, xp), scl_(rt.s, emxq));
+        Mx[i][rt.q] = add_(scl_(rt.s, xp), scl_(rt.c, emxq));
+      }
+      __syncthreads();
+      // rows of G:  [g_p; g_q] <- J^H [g_p; g_q]
+      for (int idx = tid; idx < (P2 / 2) * w; idx += kThreads) {
+        const int k = idx / w, j = idx - k * w;
+        const JRot rt = sRot[k];
+        if (rt.p < 0) continue;
+        const T ep = conj_(sEm[k]);
+        const T gp = sG[rt.p][j], gq = sG[rt.q][j];
+        const T epgq = mul_(ep, gq);
+        T np = sub_(scl_(rt.c, gp), scl_(rt.s, epgq));
+        T nq = add_(scl_(rt.s, gp), scl_(rt.c, epgq));
+        if (j == rt.q) np = zero_<T>();
+        if (j == rt.p) nq = zero_<T>();
+        sG[rt.p][j] = np;
+        sG[rt.q][j] = nq;
+      }
+      __syncthreads();
+    }
+    if (!sAny) break;
+    __syncthreads();
+  }
+  __syncthreads();
+
+  // ---- 4. P <- P Q and V_panel <- V_panel Q ----
+  for (int pass = 0; pass < 2; ++pass) {
+    T *base = pass ? V : X;
+    const int64_t rows = pass ? mt.N : mt.M;
+    for (int64_t r0 = 0; r0 < rows; r0 += kRC) {
+      for (int idx = tid; idx < kRC * w; idx += kThreads) {
+        const int c = idx / kRC, rr = idx - c * kRC;
+        const int64_t row = r0 + rr;
+        sP[rr][c] = row < rows ? base[int64_t(sCol[c]) * rows + row] : zero_<T>();
+      }
+      __syncthreads();
+      for (int idx = tid; idx < kRC * w; idx += kThreads) {
+        const int j = idx / kRC, rr = idx - j * kRC;
+        const int64_t row = r0 + rr;
+        if (row < rows) {
+          T s = zero_<T>();
+          for (int c = 0; c < w; ++c) s = add_(s, mul_(sP[rr][c], sQ[c][j]));
+          base[int64_t(sCol[j]) * rows + row] = s;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_finish(const SvdMat *mats, const void *ws_, void *u_, double *s_,
+                                                          void *vt_) {
+  using T = typename ElemOf<C>::T;
+  __shared__ double sn[kMaxSortCols];
+  __shared__ int32_t so[kMaxSortCols];
+  const SvdMat mt = mats[blockIdx.x];
+  const T *X = static_cast<const T *>(ws_) + mt.x_off;
+  const T *V = static_cast<const T *>(ws_) + mt.v_off;
+  T *U = static_cast<T *>(u_) + mt.u_off;
+  T *Vt = static_cast<T *>(vt_) + mt.vt_off;
+  double *S = s_ + mt.s_off;
+  norms_and_order(X, mt.M, mt.N, sn, so);
+  const int64_t r = mt.N;
+  for (int64_t k = threadIdx.x; k < r; k += kThreads) S[k] = sqrt(sn[so[k]]);
+  if (!mt.tr) {
+    // A = X_final V^H:  U (m x r) = X_col / sigma, Vt (r x n) = V^H
+    for (int64_t e = threadIdx.x; e < mt.m * r; e += kThreads) {
+      const int64_t i = e / r, k = e - i * r;
+      const int32_t c = so[k];
+      const double sg = sqrt(sn[c]);
+      U[e] = sg > 0.0 ? scl_(1.0 / sg, X[int64_t(c) * mt.M + i]) : zero_<T>();
+    }
+    for (int64_t e = threadIdx.x; e < r * mt.n; e += kThreads) {
+      const int64_t k = e / mt.n, j = e - k * mt.n;
+      Vt[e] = conj_(V[int64_t(so[k]) * mt.N + j]);
+    }
+  } else {
+    // A^H = X_final V^H  =>  A = V X_final^H:  U (m x r) = V, Vt (r x n) = (X_col / sigma)^H
+    for (int64_t e = threadIdx.x; e < mt.m * r; e += kThreads) {
+      const int64_t i = e / r, k = e - i * r;
+      U[e] = V[int64_t(so[k]) * mt.N + i];
+    }
+    for (int64_t e = threadIdx.x; e < r * mt.n; e += kThreads) {
+      const int64_t k = e / mt.n, j = e - k * mt.n;
+      const int32_t c = so[k];
+      const double sg = sqrt(sn[c]);
+      Vt[e] = sg > 0.0 ? scl_(1.0 / sg, conj_(X[int64_t(c) * mt.M + j])) : zero_<T>();
+    }
+  }
+}
+
+// ---- QR ----
+
+struct QrMat {
+  int64_t m, n, k;
+  int64_t w_off;      // W: column-major m x n (becomes R and the reflectors)
+  int64_t q_off_ws;   // Qw: column-major m x k
+  int64_t t_off;      // tau (k elements)
+  int64_t a_off, q_off, r_off;
+};
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double s = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return s;
+}
+
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_qr(const QrMat *mats, const void *a_, void *ws_, void *q_, void *r_) {
+  using T = typename ElemOf<C>::T;
+  __shared__ double sRed[kWaves];
+  __shared__ T sTau, sScale;
+  const QrMat mt = mats[blockIdx.x];
+  const T *A = static_cast<const T *>(a_) + mt.a_off;
+  T *Wm = static_cast<T *>(ws_) + mt.w_off;
+  T *Qw = static_cast<T *>(ws_) + mt.q_off_ws;
+  T *tau = static_cast<T *>(ws_) + mt.t_off;
+  const int64_t m = mt.m, n = mt.n, k = mt.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t e = threadIdx.x; e < m * n; e += kThreads) {
+    const int64_t j = e / m, i = e - j * m;
+    Wm[e] = A[i * n + j];
+  }
+  __syncthreads();
+  for (int64_t j = 0; j < k; ++j) {
+    T *col = Wm + j * m;
+    double part = 0.0;
+    for (int64_t i = j + 1 + threadIdx.x; i < m; i += kThreads) part += abs2_(col[i]);
+    const double xn2 = block_sum<T>(part, sRed);
+    if (threadIdx.x == 0) {
+      const T alpha = col[j];
+      T t = zero_<T>(), sc = zero_<T>();
+      const double ai = C ? (abs2_(alpha) - re_(alpha) * re_(alpha)) : 0.0;   // (Im alpha)^2
+      if (!(xn2 == 0.0 && ai == 0.0)) {
+        const double nrm = sqrt(abs2_(alpha) + xn2);
+        const double beta = re_(alpha) >= 0.0 ? -nrm : nrm;
+        T bt = zero_<T>();
+        bt = add_(bt, scl_(beta, one_<T>()));
+        t = scl_(1.0 / beta, sub_(bt, alpha));
+        sc = recip_(sub_(alpha, bt));
+        col[j] = bt;
+      }
+      sTau = t;
+      sScale = sc;
+      tau[j] = t;
+    }
+    __syncthreads();
+    const T t = sTau;
+    if (abs2_(t) == 0.0) continue;
+    const T sc = sScale;
+    for (int64_t i = j + 1 + threadIdx.x; i < m; i += kThreads) col[i] = mul_(sc, col[i]);
+    __syncthreads();
+    // trailing columns: x <- H^H x = x - conj(tau) v (v^H x)
+    const T ct = conj_(t);
+    for (int64_t c = j + 1 + wave; c < n; c += kWaves) {
+      T *x = Wm + c * m;
+      T d = zero_<T>();
+      for (int64_t i = j + lane; i < m; i += 64) {
+        const T vi = (i == j) ? one_<T>() : col[i];
+        d = add_(d, cmul_(vi, x[i]));
+      }
+      if constexpr (C) {
+        d.x = wave_sum(d.x);
+        d.y = wave_sum(d.y);
+      } else {
+        d = wave_sum(d);
+      }
+      const T f = mul_(ct, d);
+      for (int64_t i = j + lane; i < m; i += 64) {
+        const T vi = (i == j) ? one_<T>() : col[i];
+        x[i] = sub_(x[i], mul_(vi, f));
+      }
+    }
+    __syncthreads();
+  }
+  // R (k x n), upper triangle of W
+  T *R = static_cast<T *>(r_) + mt.r_off;
+  for (int64_t e = threadIdx.x; e < k * n; e += kThreads) {
+    const int64_t i = e / n, c = e - i * n;
+    R[e] = c >= i ? Wm[c * m + i] : zero_<T>();
+  }
+  // Q = H_0 ... H_{k-1} I[:, :k]
+  for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+    const int64_t c = e / m, i = e - c * m;
+    Qw[e] = (i == c) ? one_<T>() : zero_<T>();
+  }
+  __syncthreads();
+  for (int64_t j = k - 1; j >= 0; --j) {
+    const T t = tau[j];
+    if (abs2_(t) == 0.0) continue;
+    const T *v = Wm + j * m;
+    for (int64_t c = j + wave; c < k; c += kWaves) {
+      T *x = Qw + c * m;
+      T d = zero_<T>();
+      for (int64_t i = j + lane; i < m; i += 64) {
+        const T vi = (i == j) ? one_<T>() : v[i];
+        d = add_(d, cmul_(vi, x[i]));
+      }
+      if constexpr (C) {
+        d.x = wave_sum(d.x);
+        d.y = wave_sum(d.y);
+      } else {
+        d = wave_sum(d);
+      }
+      const T f = mul_(t, d);
+      for (int64_t i = j + lane; i < m; i += 64) {
+        const T vi = (i == j) ? one_<T>() : v[i];
+        x[i] = sub_(x[i], mul_(vi, f));
+      }
+    }
+    __syncthreads();
+  }
+  T *Q = static_cast<T *>(q_) + mt.q_off;
+  for (int64_t e = threadIdx.x; e < m * k; e += kThreads) {
+    const int64_t i = e / k, c = e - i * k;
+    Q[e] = Qw[c * m + i];
+  }
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p != nullptr) (void)hipFree(p);
+  }
+};
+
+qlt_status check_shapes(const char *who, int64_t n_mats, const int64_t *rows, const int64_t *cols) {
+  QLT_CHECK_ARG(n_mats >= 0, std::string(who) + ": negative matrix count");
+  QLT_CHECK_ARG(n_mats == 0 || (rows != nullptr && cols != nullptr), std::string(who) + ": null shape array");
+  QLT_CHECK_ARG(n_mats < (int64_t(1) << 31), std::string(who) + ": too many matrices");
+  for (int64_t i = 0; i < n_mats; ++i)
+    QLT_CHECK_ARG(rows[i] >= 1 && cols[i] >= 1, std::string(who) + ": matrix " + std::to_string(i) + " is empty");
+  return QLT_OK;
+}
+
+template <bool C>
+qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, const void *a, const int64_t *a_offsets,
+                    void *u, const int64_t *u_offsets, double *s, const int64_t *s_offsets, void *vt,
+                    const int64_t *vt_offsets, int32_t *sweeps_out, hipStream_t st) {
+  const size_t esz = C ? 16 : 8;
+  QLT_CHECK_ARG(n_mats <= 65535, "qlt_block_svd: more than 65535 matrices in one call");
+  std::vector<SvdMat> mats(static_cast<size_t>(n_mats));
+  std::vector<SvdItem> items;
+  int64_t ws_elems = 0, perm_elems = 0, max_elems = 0;
+  int32_t max_players = 0;
+  for (int64_t i = 0; i < n_mats; ++i) {
+    SvdMat &mt = mats[static_cast<size_t>(i)];
+    mt.m = rows[i];
+    mt.n = cols[i];
+    mt.tr = rows[i] < cols[i];
+    mt.M = std::max(rows[i], cols[i]);
+    mt.N = std::min(rows[i], cols[i]);
+    QLT_CHECK_ARG(mt.N <= kMaxSortCols, "qlt_block_svd: min(rows, cols) of matrix " + std::to_string(i) +
+                                            " exceeds " + std::to_string(kMaxSortCols));
+    mt.x_off = ws_elems;
+    ws_elems += mt.M * mt.N;
+    mt.v_off = ws_elems;
+    ws_elems += mt.N * mt.N;
+    mt.p_off = perm_elems;
+    perm_elems += mt.N;
+    mt.a_off = a_offsets[i];
+    mt.u_off = u_offsets[i];
+    mt.s_off = s_offsets[i];
+    mt.vt_off = vt_offsets[i];
+    mt.nb = static_cast<int32_t>((mt.N + kB - 1) / kB);
+    mt.P = mt.nb == 1 ? 2 : mt.nb + (mt.nb & 1);
+    max_elems = std::max(max_elems, mt.M * mt.N);
+    if (mt.N >= 2) {
+      max_players = std::max(max_players, mt.P);
+      for (int32_t k = 0; k < mt.P / 2; ++k) items.push_back(SvdItem{static_cast<int32_t>(i), k});
+    }
+  }
+  DevBuf d_ws, d_mats, d_items, d_perm, d_flags;
+  QLT_HIP(hipMalloc(&d_ws.p, std::max<size_t>(1, static_cast<size_t>(ws_elems) * esz)));
+  QLT_HIP(hipMalloc(&d_mats.p, sizeof(SvdMat) * mats.size()));
+  QLT_HIP(hipMalloc(&d_items.p, sizeof(SvdItem) * std::max<size_t>(1, items.size())));
+  QLT_HIP(hipMalloc(&d_perm.p, sizeof(int32_t) * std::max<int64_t>(1, perm_elems)));
+  QLT_HIP(hipMalloc(&d_flags.p, sizeof(int32_t) * 2 * mats.size()));
+  auto *active = static_cast<int32_t *>(d_flags.p);
+  auto *rotated = active + mats.size();
+  QLT_HIP(hipMemcpyAsync(d_mats.p, mats.data(), sizeof(SvdMat) * mats.size(), hipMemcpyHostToDevice, st));
+  if (!items.empty())
+    QLT_HIP(hipMemcpyAsync(d_items.p, items.data(), sizeof(SvdItem) * items.size(), hipMemcpyHostToDevice, st));
+  const auto *dm = static_cast<const SvdMat *>(d_mats.p);
+  auto *perm = static_cast<int32_t *>(d_perm.p);
+  const unsigned gx = static_cast<unsigned>(std::min<int64_t>(1024, std::max<int64_t>(1, (max_elems + kThreads - 1) / kThreads)));
+  hipLaunchKernelGGL(k_svd_load<C>, dim3(gx, static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, a,
+                     d_ws.p, perm);
+  QLT_HIP(hipGetLastError());
+  std::vector<int32_t> h_active(mats.size()), h_rot(mats.size(), 0);
+  for (size_t i = 0; i < mats.size(); ++i) h_active[i] = mats[i].N >= 2;
+  int32_t sweeps = 0;
+  bool any = !items.empty();
+  while (any && sweeps < kMaxSweeps) {
+    QLT_HIP(hipMemcpyAsync(active, h_active.data(), sizeof(int32_t) * mats.size(), hipMemcpyHostToDevice, st));
+    std::fill(h_rot.begin(), h_rot.end(), 0);
+    QLT_HIP(hipMemcpyAsync(rotated, h_rot.data(), sizeof(int32_t) * mats.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_svd_sort<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm,
+                       d_ws.p, perm, active);
+    for (int32_t r = 0; r < max_players - 1; ++r)
+      hipLaunchKernelGGL(k_svd_round<C>, dim3(static_cast<unsigned>(items.size())), dim3(kThreads), 0, st, dm,
+                         static_cast<const SvdItem *>(d_items.p), r, d_ws.p, perm, active, rotated);
+    QLT_HIP(hipGetLastError());
+    QLT_HIP(hipMemcpyAsync(h_rot.data(), rotated, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost, st));
+    QLT_HIP(hipStreamSynchronize(st));
+    ++sweeps;
+    any = false;
+    for (size_t i = 0; i < mats.size(); ++i) {
+      h_active[i] = h_active[i] && h_rot[i];
+      any = any || h_active[i];
+    }
+  }
+  hipLaunchKernelGGL(k_svd_finish<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, d_ws.p, u, s,
+                     vt);
+  QLT_HIP(hipGetLastError());
+  QLT_HIP(hipStreamSynchronize(st));
+  if (sweeps_out != nullptr) *sweeps_out = sweeps;
+  if (any) return qlt::fail(QLT_ERR_NOT_CONVERGED, "qlt_block_svd: Jacobi sweeps did not converge in " +
+                                                        std::to_string(kMaxSweeps) + " sweeps");
+  return QLT_OK;
+}
+
+template <bool C>
+qlt_status qr_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, const void *a, const int64_t *a_offsets,
+                   void *q, const int64_t *q_offsets, void *r, const int64_t *r_offsets, hipStream_t st) {
+  const size_t esz = C ? 16 : 8;
+  std::vector<QrMat> mats(static_cast<size_t>(n_mats));
+  int64_t ws = 0;
+  for (int64_t i = 0; i < n_mats; ++i) {
+    QrMat &mt = mats[static_cast<size_t>(i)];
+    mt.m = rows[i];
+    mt.n = cols[i];
+    mt.k = std::min(rows[i], cols[i]);
+    mt.w_off = ws;
+    ws += mt.m * mt.n;
+    mt.q_off_ws = ws;
+    ws += mt.m * mt.k;
+    mt.t_off = ws;
+    ws += mt.k;
+    mt.a_off = a_offsets[i];
+    mt.q_off = q_offsets[i];
+    mt.r_off = r_offsets[i];
+  }
+  DevBuf d_ws, d_mats;
+  QLT_HIP(hipMalloc(&d_ws.p, std::max<size_t>(1, static_cast<size_t>(ws) * esz)));
+  QLT_HIP(hipMalloc(&d_mats.p, sizeof(QrMat) * mats.size()));
+  QLT_HIP(hipMemcpyAsync(d_mats.p, mats.data(), sizeof(QrMat) * mats.size(), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_qr<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st,
+                     static_cast<const QrMat *>(d_mats.p), a, d_ws.p, q, r);
+  QLT_HIP(hipGetLastError());
+  QLT_HIP(hipStreamSynchronize(st));
+  return QLT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
-qlt_status qlt_block_svd_f64(int64_t, const int64_t *, const int64_t *, const double *, const int64_t *,
-                             double *, const int64_t *, double *, const int64_t *, double *, const int64_t *,
-                             void *) {
-  return qlt::fail(QLT_ERR_UNSUPPORTED, "qlt_block_svd_f64 is not built yet");
+qlt_status qlt_block_svd(int32_t dtype, int64_t n_mats, const int64_t *rows, const int64_t *cols, const void *a,
+                         const int64_t *a_offsets, void *u, const int64_t *u_offsets, double *s,
+                         const int64_t *s_offsets, void *vt, const int64_t *vt_offsets, int32_t *sweeps,
+                         void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_block_svd: bad dtype");
+  QLT_TRY(check_shapes("qlt_block_svd", n_mats, rows, cols));
+  if (sweeps != nullptr) *sweeps = 0;
+  if (n_mats == 0) return QLT_OK;
+  QLT_CHECK_ARG(a != nullptr && u != nullptr && s != nullptr && vt != nullptr, "qlt_block_svd: null data pointer");
+  QLT_CHECK_ARG(a_offsets != nullptr && u_offsets != nullptr && s_offsets != nullptr && vt_offsets != nullptr,
+                "qlt_block_svd: null offset array");
+  QLT_CHECK_ARG(!qlt::stream_capturing(qlt::as_stream(stream)), "qlt_block_svd: not capturable (host-driven sweeps)");
+  if (dtype == QLT_F64)
+    return svd_impl<false>(n_mats, rows, cols, a, a_offsets, u, u_offsets, s, s_offsets, vt, vt_offsets, sweeps,
+                           qlt::as_stream(stream));
+  return svd_impl<true>(n_mats, rows, cols, a, a_offsets, u, u_offsets, s, s_offsets, vt, vt_offsets, sweeps,
+                        qlt::as_stream(stream));
 }
 
-qlt_status qlt_block_qr_f64(int64_t, const int64_t *, const int64_t *, const double *, const int64_t *,
-                            double *, const int64_t *, double *, const int64_t *, void *) {
-  return qlt::fail(QLT_ERR_UNSUPPORTED, "qlt_block_qr_f64 is not built yet");
+qlt_status qlt_block_qr(int32_t dtype, int64_t n_mats, const int64_t *rows, const int64_t *cols, const void *a,
+                        const int64_t *a_offsets, void *q, const int64_t *q_offsets, void *r,
+                        const int64_t *r_offsets, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_block_qr: bad dtype");
+  QLT_TRY(check_shapes("qlt_block_qr", n_mats, rows, cols));
+  if (n_mats == 0) return QLT_OK;
+  QLT_CHECK_ARG(a != nullptr && q != nullptr && r != nullptr, "qlt_block_qr: null data pointer");
+  QLT_CHECK_ARG(a_offsets != nullptr && q_offsets != nullptr && r_offsets != nullptr,
+                "qlt_block_qr: null offset array");
+  QLT_CHECK_ARG(!qlt::stream_capturing(qlt::as_stream(stream)), "qlt_block_qr: not capturable (allocates)");
+  if (dtype == QLT_F64)
+    return qr_impl<false>(n_mats, rows, cols, a, a_offsets, q, q_offsets, r, r_offsets, qlt::as_stream(stream));
+  return qr_impl<true>(n_mats, rows, cols, a, a_offsets, q, q_offsets, r, r_offsets, qlt::as_stream(stream));
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "qlt_internal.h"
 
@@ -471,6 +472,65 @@ qlt_status qlt_conjugate(int64_t n, double *z, void *stream) {
   QLT_CHECK_ARG(z != nullptr, "qlt_conjugate: null buffer");
   hipLaunchKernelGGL(k_conjugate, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, z);
   QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Copy2D {
+  int64_t rows, cols, src_off, src_ld, dst_off, dst_ld;
+};
+
+// One workgroup per rectangle; W doubles per element.
+template <int W>
+__global__ void k_copy_2d(const Copy2D *items, const double *src, double *dst) {
+  const Copy2D it = items[blockIdx.x];
+  const int64_t n = it.rows * it.cols;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const int64_t r = e / it.cols, c = e - r * it.cols;
+    const int64_t si = (it.src_off + r * it.src_ld + c) * W, di = (it.dst_off + r * it.dst_ld + c) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) dst[di + w] = src[si + w];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+qlt_status qlt_copy_blocks_2d(int32_t dtype, int64_t n, const int64_t *rows, const int64_t *cols,
+                              const void *src, const int64_t *src_offsets, const int64_t *src_ld,
+                              void *dst, const int64_t *dst_offsets, const int64_t *dst_ld, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_copy_blocks_2d: bad dtype");
+  QLT_CHECK_ARG(n >= 0 && n < (int64_t(1) << 31), "qlt_copy_blocks_2d: bad rectangle count");
+  if (n == 0) return QLT_OK;
+  QLT_CHECK_ARG(rows && cols && src_offsets && src_ld && dst_offsets && dst_ld, "qlt_copy_blocks_2d: null array");
+  QLT_CHECK_ARG(src != nullptr && dst != nullptr, "qlt_copy_blocks_2d: null data pointer");
+  hipStream_t st = as_stream(stream);
+  QLT_CHECK_ARG(!stream_capturing(st), "qlt_copy_blocks_2d: not capturable (stages host arrays)");
+  std::vector<Copy2D> h(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    QLT_CHECK_ARG(rows[i] >= 0 && cols[i] >= 0 && src_ld[i] >= cols[i] && dst_ld[i] >= cols[i] &&
+                      src_offsets[i] >= 0 && dst_offsets[i] >= 0,
+                  "qlt_copy_blocks_2d: bad rectangle " + std::to_string(i));
+    h[static_cast<size_t>(i)] = Copy2D{rows[i], cols[i], src_offsets[i], src_ld[i], dst_offsets[i], dst_ld[i]};
+  }
+  Copy2D *d = nullptr;
+  QLT_HIP(hipMallocAsync(reinterpret_cast<void **>(&d), sizeof(Copy2D) * h.size(), st));
+  QLT_HIP(hipMemcpyAsync(d, h.data(), sizeof(Copy2D) * h.size(), hipMemcpyHostToDevice, st));
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL(k_copy_2d<1>, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
+                       static_cast<const double *>(src), static_cast<double *>(dst));
+  else
+    hipLaunchKernelGGL(k_copy_2d<2>, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
+                       static_cast<const double *>(src), static_cast<double *>(dst));
+  QLT_HIP(hipGetLastError());
+  QLT_HIP(hipStreamSynchronize(st));
+  QLT_HIP(hipFreeAsync(d, st));
+  QLT_HIP(hipStreamSynchronize(st));
   return QLT_OK;
 }
 

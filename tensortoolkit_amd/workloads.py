@@ -35,6 +35,8 @@ MATVEC_AXES = [
 def largest_remainder(weights, total: int) -> np.ndarray:
     """Integers proportional to `weights` summing exactly to `total` (each >= 1)."""
     w = np.asarray(weights, dtype=np.float64)
+    if total < len(w):
+        raise ValueError(f"cannot give each of {len(w)} sectors a degeneracy >= 1 from a total of {total}")
     raw = w / w.sum() * total
     base = np.floor(raw).astype(np.int64)
     base = np.maximum(base, 1)
@@ -148,3 +150,50 @@ def small_matvec(seed: int = 7, D: int = 96, dtype: str = "f64", u1u1: bool = Fa
     charges, degs = u1_gaussian_degs(3, D, var=3.0)
     return build_matvec([(int(q),) for q in charges], degs, [(0,), (1,)], HEIS_MPO_QNS, HEIS_MPO_DEGS,
                         dtype, seed, f"small_u1_D{D}")
+
+
+def _partner_sector(ix: Index) -> np.ndarray:
+    """Sector of -q for every sector q of an MPO bond index (the conjugate
+    operator's bond sector, e.g. S+ <-> S-)."""
+    pos = {tuple(q): k for k, q in enumerate(ix.qns)}
+    out = np.empty(ix.nsec, dtype=np.int64)
+    for k, q in enumerate(ix.qns):
+        partner = pos[tuple(-v for v in q)]
+        assert ix.degs[partner] == ix.degs[k], "conjugate MPO sectors need equal degeneracies"
+        out[k] = partner
+    return out
+
+
+def _symmetrize(t: BlockSparseTensor, coord_map, axes_perm) -> BlockSparseTensor:
+    """t'[block c] = (t[c] + conj(t[pi(c)]).transpose(axes_perm)) / 2, with
+    pi(c) = coord_map(c); the partner block must exist."""
+    lut = {tuple(int(x) for x in t.coords[i]): i for i in range(t.n_blks)}
+    data = np.empty_like(t.data)
+    for i in range(t.n_blks):
+        j = lut[coord_map(tuple(int(x) for x in t.coords[i]))]
+        x = t.block(i)
+        y = np.conj(t.block(j)).transpose(axes_perm)
+        data[t.offsets[i]:t.offsets[i] + x.size] = (0.5 * (x + y)).reshape(-1)
+    return BlockSparseTensor(t.indexes, t.coords, t.dims, dtype=t.dtype, data=data, offsets=t.offsets)
+
+
+def hermitian_matvec(mv: Matvec) -> Matvec:
+    """The same tensors made to give a Hermitian effective Hamiltonian
+    H = sum L_w (x) W_ww' (x) W_w'w'' (x) R_w'': with w~ the conjugate MPO
+    sector, L[a',w,a] = conj L[a,w~,a'], R[b,w,b'] = conj R[b',w~,b] and
+    W[wl,wr,so,si] = conj W[wl~,wr~,si,so], each imposed by averaging a block
+    with its conjugate partner.  Every term of H is then a product of
+    operators whose sum over the bond indices is Hermitian."""
+    wbar_l = _partner_sector(mv.L.indexes[1])
+    wbar_r = _partner_sector(mv.R.indexes[1])
+    wbar_w = _partner_sector(mv.W.indexes[0])
+    L = _symmetrize(mv.L, lambda c: (c[2], int(wbar_l[c[1]]), c[0]), (2, 1, 0))
+    R = _symmetrize(mv.R, lambda c: (c[2], int(wbar_r[c[1]]), c[0]), (2, 1, 0))
+    W = _symmetrize(mv.W, lambda c: (int(wbar_w[c[0]]), int(wbar_w[c[1]]), c[3], c[2]), (0, 1, 3, 2))
+    return Matvec(L=L, psi=mv.psi, W=W, R=R, dtype=mv.dtype, name=mv.name + "_hermitian")
+
+
+def config4(seed: int = 20261202, D: int = 4096) -> Matvec:
+    """End-to-end two-site update on the U(1)xU(1) D=4096 complex128 tensors
+    with L, W, R made Hermitian (Lanczos needs a Hermitian H_eff)."""
+    return hermitian_matvec(config3(seed, D))

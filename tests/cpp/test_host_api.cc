@@ -5,6 +5,7 @@
 // against a dense contraction computed on the host; path equivalence of the
 // contiguous-axes API against the general Contract; accumulate semantics.
 // Exit status 0 means every check passed.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -124,6 +125,13 @@ double RelDiff(const std::vector<T> &x, const std::vector<T> &y) {
     den += std::norm(std::complex<double>(y[i]));
   }
   return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+}
+
+// S is real; the complex tests contract it after ToComplex.
+template <typename T>
+QLTensor<T, U1QN> ToComplexIfNeeded(const QLTensor<QLTEN_Double, U1QN> &s) {
+  if constexpr (IsComplex<T>()) return ToComplex(s);
+  else return s;
 }
 
 struct Fixture {
@@ -331,6 +339,74 @@ int main() {
     CHECK(RelDiff(Densify(u), [&] { auto d = Densify(t); for (auto &v : d) v *= 3.0; return d; }()) < 1e-14,
           "operator+ / operator*");
     CHECK(t.Div() == U1QN(0), "Div");
+  }
+
+  // ---- block SVD and QR (ten_svd.h / ten_qr.h semantics) ----
+  {
+    // known answer: tests/package_consumer/smoke_gpu.cu:100-117
+    QNSectorVec<U1QN> one = {QNSctT(U1QN(0), 2)};
+    DTen t({U1Idx(one, IN), U1Idx(one, OUT)});
+    t.Fill(U1QN(0), 0.0);
+    t.SetElem({0, 0}, 1.0);
+    t.SetElem({0, 1}, 2.0);
+    t.SetElem({1, 0}, 3.0);
+    t.SetElem({1, 1}, 4.0);
+    DTen u, vt;
+    QLTensor<QLTEN_Double, U1QN> s;
+    double err = -1;
+    size_t D = 0;
+    SVD(&t, 1, U1QN(0), 0.0, 1, 10, &u, &s, &vt, &err, &D);
+    CHECK(D == 2 && err == 0.0, "svd 2x2 D / err");
+    CHECK(std::abs(s.GetElem({0, 0}) - 5.464985704219043) < 1e-14 * 5.5, "svd 2x2 sigma_0");
+    CHECK(std::abs(s.GetElem({1, 1}) - 0.365966190626257) < 1e-14 * 5.5, "svd 2x2 sigma_1");
+  }
+  auto svd_roundtrip = [&](auto tag, double trunc, const std::string &name) {
+    using T = decltype(tag);
+    QLTensor<T, U1QN> t({f.idx_in_l, f.idx_in_s, f.idx_out_s, f.idx_out_l});
+    t.Random(U1QN(0));
+    QLTensor<T, U1QN> u, vt, us, back;
+    QLTensor<QLTEN_Double, U1QN> s;
+    double err = -1;
+    size_t D = 0;
+    SVD(&t, 2, U1QN(0), trunc, 1, 100000, &u, &s, &vt, &err, &D);
+    CHECK(D == s.GetShape()[0] && u.GetShape()[2] == D && vt.GetShape()[0] == D, name + ": D");
+    QLTensor<T, U1QN> sc = ToComplexIfNeeded<T>(s);
+    Contract(&u, &sc, {{2}, {0}}, &us);
+    Contract(&us, &vt, {{2}, {0}}, &back);
+    const auto dt = Densify(t), db = Densify(back);
+    const double rel = RelDiff(db, dt);
+    if (trunc == 0.0) {
+      CHECK(rel < 1e-12, name + ": U S Vt == T, rel " + std::to_string(rel));
+      CHECK(err < 1e-14, name + ": no truncation error");
+    } else {
+      CHECK(std::abs(rel * rel - err) < 1e-10, name + ": |T - U S Vt|^2 / |T|^2 == err");
+    }
+    QLTensor<T, U1QN> ud = Dag(u), uu;
+    Contract(&ud, &u, {{0, 1}, {0, 1}}, &uu);
+    const auto duu = Densify(uu);
+    double dev = 0;
+    for (size_t i = 0; i < D; ++i)
+      for (size_t j = 0; j < D; ++j) dev = std::max(dev, std::abs(duu[i * D + j] - T(i == j ? 1.0 : 0.0)));
+    CHECK(dev < 1e-12, name + ": U^dag U == 1");
+  };
+  svd_roundtrip(QLTEN_Double(0), 0.0, "svd real");
+  svd_roundtrip(QLTEN_Complex(0), 0.0, "svd complex");
+  svd_roundtrip(QLTEN_Double(0), 1e-2, "svd real truncated");
+  {
+    ZTen t({f.idx_in_l, f.idx_in_s, f.idx_out_s, f.idx_out_l});
+    t.Random(U1QN(0));
+    ZTen q, r, back;
+    QR(&t, 2, U1QN(0), &q, &r);
+    Contract(&q, &r, {{2}, {0}}, &back);
+    CHECK(RelDiff(Densify(back), Densify(t)) < 1e-12, "qr: Q R == T");
+    ZTen qd = Dag(q), qq;
+    Contract(&qd, &q, {{0, 1}, {0, 1}}, &qq);
+    const size_t k = q.GetShape()[2];
+    const auto dq = Densify(qq);
+    double dev = 0;
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < k; ++j) dev = std::max(dev, std::abs(dq[i * k + j] - QLTEN_Complex(i == j ? 1.0 : 0.0)));
+    CHECK(dev < 1e-12, "qr: Q^dag Q == 1");
   }
 
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
