@@ -22,8 +22,9 @@
  *                        plain transpose), one GEMM per task C = f*A'B' + beta*C,
  *                        row-major NN with lda=k, ldb=n, ldc=n
  *                        (framework/hp_numeric/blas_level3.h:35-108).
- *   ora_execute_ordered  global_operations.h:994-1080 (CtrctAccordingTask):
- *                        tasks in generation order, no sort.
+ *                        With sorted == 0 the same function runs the tasks in
+ *                        generation order without the sort, as
+ *                        CtrctAccordingTask (global_operations.h:994-1080).
  * Pinning (each checked in tests/test_oracle.py):
  *   - plan: brute-force enumeration of all block pairs by coordinates on tiny
  *     inputs; the reference's cost known-answer test
