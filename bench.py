@@ -330,6 +330,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="both", choices=["both", "native", "split"],
+                    help="contraction modes to time (the JSON line needs both for split_vs_native)")
     ap.add_argument("--graph-replays", type=int, default=100)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (scales rocprofv3 "
@@ -370,6 +372,8 @@ def main() -> int:
 
     results = {}
     modes = [("native", capi.QLT_MODE_NATIVE), ("split_bf16", capi.QLT_MODE_SPLIT_BF16)]
+    if args.mode != "both":
+        modes = [m for m in modes if m[0].startswith(args.mode)]
     outputs = {}
     dom = int(np.argmax(runner.flops))
     for name, mode in modes:

@@ -25,7 +25,23 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 16;   // k-chunk staged through LDS per pipeline step
+constexpr int KC = kKChunk;   // k-chunk staged through LDS per pipeline step
+
+// Tile order for the XCDs: workgroups are dispatched to the 8 XCDs round
+// robin, so workgroup b runs on XCD b % 8.  Mapping b to logical tile
+// (r / G) * 8G + (b % 8) * G + r % G, with r = b / 8, gives each XCD runs of G
+// consecutive tiles (neighbouring tiles of one output block, which share A
+// rows or B columns) while keeping the global longest-first order at a
+// granularity of 8G tiles; the tail beyond the last full 8G group maps to
+// itself.
+constexpr int kXcds = 8;
+constexpr int kXcdRun = 4;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  constexpr int span = kXcds * kXcdRun;
+  if (b >= (n / span) * span) return b;
+  const int r = b / kXcds;
+  return (r / kXcdRun) * span + (b % kXcds) * kXcdRun + (r % kXcdRun);
+}
 
 // Offset of linear index `x` inside a block, decomposed row-major over
 // dims[0..n-1] (last fastest) with the given memory strides.
@@ -53,15 +69,15 @@ template <bool CPLX, int TMv, int TNv>
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
-    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
-    double alpha_i, double beta_r, double beta_i) {
+    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
+    double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   constexpr int QA = TMv / 16, QB = TNv / 16;          // elements per thread per chunk
   constexpr int BM = TMv / 32, BN = TNv / 32;          // MFMA blocks per wave
   __shared__ double As[2][P][KC][TMv + 1];
   __shared__ double Bs[2][P][KC][TNv + 1];
 
-  const Tile tile = tiles[blockIdx.x];
+  const Tile tile = tiles[xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const Seg seg = segs[tile.seg];
   const int i0 = tile.tm * TMv, j0 = tile.tn * TNv;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -144,21 +160,21 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     }
   };
 
-  // ---- pipeline over the (task, k-chunk) sequence of this output block ----
-  int t = seg.t0, kk = 0;
+  // ---- pipeline over this tile's steps of the (task, k-chunk) sequence ----
+  int t = tile.t_begin, kk = tile.k_begin;
   ExecTask et = exec[t];
   task_rows(ga[et.a_blk], gb[et.b_blk], et);
-  fetch(et, ga[et.a_blk], gb[et.b_blk], 0);
+  fetch(et, ga[et.a_blk], gb[et.b_blk], kk);
   stage(0);
   __syncthreads();
   int buf = 0;
-  while (t < seg.t1) {
+  for (int step = 0; step < tile.n_chunks; ++step) {
     int t2 = t, kk2 = kk + KC;
     if (kk2 >= et.k) {
       t2 = t + 1;
       kk2 = 0;
     }
-    const bool has_next = t2 < seg.t1;
+    const bool has_next = step + 1 < tile.n_chunks;
     ExecTask et2 = et;
     if (has_next) {
       if (t2 != t) {
@@ -204,6 +220,22 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     buf ^= 1;
   }
 
+  // ---- a k-slice stores its unscaled partial sum, in the thread's own order ----
+  if (tile.part >= 0) {
+    double *pp = part + static_cast<int64_t>(tile.part) * (TMv * TNv * P);
+#pragma unroll
+    for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int e = ((mt * BN + nt) * 4 + v) * 256 + tid;
+          pp[e] = acc[mt][nt][v];
+          if (CPLX) pp[TMv * TNv + e] = acc_i[mt][nt][v];
+        }
+    return;
+  }
+
   // ---- epilogue: C = alpha * acc + beta * C (C untouched when beta == 0) ----
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 #pragma unroll
@@ -235,6 +267,76 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       }
 }
 
+// Sum of the k-slices of one output tile, in slice order, then the usual
+// epilogue.  The thread-to-element map is the MFMA kernel's, so thread tid
+// reads exactly the partial elements it would have held.
+template <bool CPLX, int TMv, int TNv>
+__global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restrict__ items,
+                                                       const Seg *__restrict__ segs,
+                                                       const double *__restrict__ part, double *__restrict__ C,
+                                                       double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  constexpr int BM = TMv / 32, BN = TNv / 32;
+  const ReduceItem it = items[blockIdx.x];
+  const Seg seg = segs[it.seg];
+  const int i0 = it.tm * TMv, j0 = it.tn * TNv;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * (TMv / 2), wn = (w & 1) * (TNv / 2);
+  const int lr = lane & 15, lk = lane >> 4;
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  // slices outer, elements inner: every slice's loads are independent
+  double sr[BM][BN][4], si[BM][BN][4];
+#pragma unroll
+  for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sr[mt][nt][v] = 0.0;
+        si[mt][nt][v] = 0.0;
+      }
+  for (int s = 0; s < it.n_parts; ++s) {
+    const double *pp = part + static_cast<int64_t>(it.part0 + s) * (TMv * TNv * P);
+#pragma unroll
+    for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int e = ((mt * BN + nt) * 4 + v) * 256 + tid;
+          sr[mt][nt][v] += pp[e];
+          if (CPLX) si[mt][nt][v] += pp[TMv * TNv + e];
+        }
+  }
+#pragma unroll
+  for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double xr = sr[mt][nt][v], xi = si[mt][nt][v];
+        const int row = i0 + wm + mt * 16 + 4 * v + lk;
+        const int col = j0 + wn + nt * 16 + lr;
+        if (row < seg.m && col < seg.n) {
+          const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
+          if (CPLX) {
+            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+            if (use_beta) {
+              const double2 o = *cp;
+              yr += beta_r * o.x - beta_i * o.y;
+              yi += beta_r * o.y + beta_i * o.x;
+            }
+            *cp = double2{yr, yi};
+          } else {
+            double y = alpha_r * xr;
+            if (use_beta) y += beta_r * C[idx];
+            C[idx] = y;
+          }
+        }
+      }
+}
+
 // Row-streaming kernel for narrow outputs (n <= NMAX): one thread per output
 // row, RPT rows per thread (rows tid, tid + 256, ...).  A' is read straight
 // from global memory -- consecutive rows are consecutive threads, so the
@@ -246,109 +348,10 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 constexpr int RK = 16;            // k per stage
 
 template <bool CPLX, int NMAX, int RPT>
-__global__ __launch_bounds__(256) void k_rows_f64(
-    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
-    const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
-    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double alpha_r,
-    double alpha_i, double beta_r, double beta_i) {
-  constexpr int P = CPLX ? 2 : 1;
-  static_assert(RK * NMAX <= 256, "one B' element per thread");
-  __shared__ double Bs[P][RK][NMAX];
-  __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[NMAX];
-
-  const Tile tile = tiles[blockIdx.x];
-  const Seg seg = segs[tile.seg];
-  const int tid = threadIdx.x;
-  const int r0 = tile.tm * (256 * RPT);
+__device__ __forceinline__ void rows_epilogue(const Seg &seg, int r0, int tid, const double (&acc)[RPT][NMAX],
+                                              const double (&acc_i)[RPT][NMAX], double *__restrict__ C,
+                                              double alpha_r, double alpha_i, double beta_r, double beta_i) {
   const int n = seg.n;
-
-  double acc[RPT][NMAX], acc_i[RPT][NMAX];
-#pragma unroll
-  for (int q = 0; q < RPT; ++q)
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      acc[q][j] = 0.0;
-      acc_i[q][j] = 0.0;
-    }
-
-  for (int t = seg.t0; t < seg.t1; ++t) {
-    const ExecTask et = exec[t];
-    const BlkGather &gA = ga[et.a_blk];
-    const BlkGather &gB = gb[et.b_blk];
-    const bool a_plain = gA.plain != 0, b_plain = gB.plain != 0;
-    int32_t roff[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int row = r0 + tid + 256 * q;
-      roff[q] = (row < seg.m) ? (a_plain ? row * et.k : split_offset(row, gA.rdim, gA.rstr, L.a_nr)) : -1;
-    }
-    __syncthreads();
-    if (tid < NMAX)
-      s_bcol[tid] = (tid < n) ? (b_plain ? tid : split_offset(tid, gB.cdim, gB.cstr, L.b_nc)) : -1;
-    for (int kk = 0; kk < et.k; kk += RK) {
-      if (tid < RK) {
-        const int p = kk + tid;
-        s_acol[tid] = (p < et.k) ? (a_plain ? p : split_offset(p, gA.cdim, gA.cstr, L.a_nc)) : -1;
-      } else if (tid >= 64 && tid < 64 + RK) {
-        const int p = kk + tid - 64;
-        s_brow[tid - 64] = (p < et.k) ? (b_plain ? p * n : split_offset(p, gB.rdim, gB.rstr, L.b_nr)) : -1;
-      }
-      __syncthreads();
-      if (tid < RK * NMAX) {
-        const int p = tid / NMAX, j = tid % NMAX;
-        const int32_t ro = s_brow[p], co = s_bcol[j];
-        const bool ok = (ro >= 0) && (co >= 0);
-        const int64_t e = et.b_off + ro + co;
-        if (CPLX) {
-          const double2 v = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
-          Bs[0][p][j] = v.x;
-          Bs[P - 1][p][j] = v.y;
-        } else {
-          Bs[0][p][j] = ok ? B[e] : 0.0;
-        }
-      }
-      __syncthreads();
-      const int kc = min(RK, et.k - kk);
-#pragma unroll 2
-      for (int p = 0; p < kc; ++p) {
-        const int32_t co = s_acol[p];
-        double av[RPT], av_i[RPT];
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          const bool ok = roff[q] >= 0;
-          const int64_t e = et.a_off + roff[q] + co;
-          if (CPLX) {
-            const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
-            av[q] = v.x;
-            av_i[q] = v.y;
-          } else {
-            av[q] = ok ? A[e] : 0.0;
-            av_i[q] = 0.0;
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < NMAX; ++j) {
-          if (j < n) {
-            const double b = Bs[0][p][j];
-            const double b_im = Bs[P - 1][p][j];
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-              if (CPLX) {
-                acc[q][j] = fma(av[q], b, acc[q][j]);
-                acc[q][j] = fma(-av_i[q], b_im, acc[q][j]);
-                acc_i[q][j] = fma(av[q], b_im, acc_i[q][j]);
-                acc_i[q][j] = fma(av_i[q], b, acc_i[q][j]);
-              } else {
-                acc[q][j] = fma(av[q], b, acc[q][j]);
-              }
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
@@ -376,6 +379,176 @@ __global__ __launch_bounds__(256) void k_rows_f64(
       }
     }
   }
+}
+
+template <bool CPLX, int NMAX, int RPT>
+__global__ __launch_bounds__(256) void k_rows_f64(
+    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
+    const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
+    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
+    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  static_assert(RK * NMAX <= 256, "one B' element per thread");
+  static_assert(RK == kKChunk, "row kernel steps are the plan's k-chunks");
+  __shared__ double Bs[P][RK][NMAX];
+  __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[NMAX];
+
+  const Tile tile = tiles[blockIdx.x];
+  const Seg seg = segs[tile.seg];
+  const int tid = threadIdx.x;
+  const int r0 = tile.tm * (256 * RPT);
+  const int n = seg.n;
+
+  double acc[RPT][NMAX], acc_i[RPT][NMAX];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      acc[q][j] = 0.0;
+      acc_i[q][j] = 0.0;
+    }
+
+  // this tile's steps of the (task, k-chunk) sequence
+  int t = tile.t_begin, kk = tile.k_begin;
+  ExecTask et{};
+  bool a_plain = false, b_plain = false;
+  int32_t roff[RPT];
+  int cur_task = -1;
+  for (int step = 0; step < tile.n_chunks; ++step) {
+    if (t != cur_task) {
+      cur_task = t;
+      et = exec[t];
+      const BlkGather &gA = ga[et.a_blk];
+      const BlkGather &gB = gb[et.b_blk];
+      a_plain = gA.plain != 0;
+      b_plain = gB.plain != 0;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int row = r0 + tid + 256 * q;
+        roff[q] = (row < seg.m) ? (a_plain ? row * et.k : split_offset(row, gA.rdim, gA.rstr, L.a_nr)) : -1;
+      }
+      __syncthreads();
+      if (tid < NMAX)
+        s_bcol[tid] = (tid < n) ? (b_plain ? tid : split_offset(tid, gB.cdim, gB.cstr, L.b_nc)) : -1;
+    }
+    const BlkGather &gA = ga[et.a_blk];
+    const BlkGather &gB = gb[et.b_blk];
+    if (tid < RK) {
+      const int p = kk + tid;
+      s_acol[tid] = (p < et.k) ? (a_plain ? p : split_offset(p, gA.cdim, gA.cstr, L.a_nc)) : -1;
+    } else if (tid >= 64 && tid < 64 + RK) {
+      const int p = kk + tid - 64;
+      s_brow[tid - 64] = (p < et.k) ? (b_plain ? p * n : split_offset(p, gB.rdim, gB.rstr, L.b_nr)) : -1;
+    }
+    __syncthreads();
+    if (tid < RK * NMAX) {
+      const int p = tid / NMAX, j = tid % NMAX;
+      const int32_t ro = s_brow[p], co = s_bcol[j];
+      const bool ok = (ro >= 0) && (co >= 0);
+      const int64_t e = et.b_off + ro + co;
+      if (CPLX) {
+        const double2 v = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
+        Bs[0][p][j] = v.x;
+        Bs[P - 1][p][j] = v.y;
+      } else {
+        Bs[0][p][j] = ok ? B[e] : 0.0;
+      }
+    }
+    __syncthreads();
+    const int kc = min(RK, et.k - kk);
+#pragma unroll 2
+    for (int p = 0; p < kc; ++p) {
+      const int32_t co = s_acol[p];
+      double av[RPT], av_i[RPT];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const bool ok = roff[q] >= 0;
+        const int64_t e = et.a_off + roff[q] + co;
+        if (CPLX) {
+          const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
+          av[q] = v.x;
+          av_i[q] = v.y;
+        } else {
+          av[q] = ok ? A[e] : 0.0;
+          av_i[q] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        if (j < n) {
+          const double b = Bs[0][p][j];
+          const double b_im = Bs[P - 1][p][j];
+#pragma unroll
+          for (int q = 0; q < RPT; ++q) {
+            if (CPLX) {
+              acc[q][j] = fma(av[q], b, acc[q][j]);
+              acc[q][j] = fma(-av_i[q], b_im, acc[q][j]);
+              acc_i[q][j] = fma(av[q], b_im, acc_i[q][j]);
+              acc_i[q][j] = fma(av_i[q], b, acc_i[q][j]);
+            } else {
+              acc[q][j] = fma(av[q], b, acc[q][j]);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    kk += RK;
+    if (kk >= et.k) {
+      ++t;
+      kk = 0;
+    }
+  }
+
+  if (tile.part >= 0) {
+    // k-slice: unscaled partial sums in the thread's own order
+    double *pp = part + static_cast<int64_t>(tile.part) * (256 * RPT * NMAX * P);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        const int e = (q * NMAX + j) * 256 + tid;
+        pp[e] = acc[q][j];
+        if (CPLX) pp[256 * RPT * NMAX + e] = acc_i[q][j];
+      }
+    return;
+  }
+  rows_epilogue<CPLX, NMAX, RPT>(seg, r0, tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
+}
+
+// Sum of the k-slices of one row tile, in slice order, then the epilogue.
+template <bool CPLX, int NMAX, int RPT>
+__global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restrict__ items,
+                                                     const Seg *__restrict__ segs, const double *__restrict__ part,
+                                                     double *__restrict__ C, double alpha_r, double alpha_i,
+                                                     double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  constexpr int SZ = 256 * RPT * NMAX * P;
+  const ReduceItem it = items[blockIdx.x];
+  const Seg seg = segs[it.seg];
+  const int tid = threadIdx.x;
+  double acc[RPT][NMAX], acc_i[RPT][NMAX];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      acc[q][j] = 0.0;
+      acc_i[q][j] = 0.0;
+    }
+  for (int s = 0; s < it.n_parts; ++s) {
+    const double *pp = part + static_cast<int64_t>(it.part0 + s) * SZ;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        if (j < seg.n) {
+          const int e = (q * NMAX + j) * 256 + tid;
+          acc[q][j] += pp[e];
+          if (CPLX) acc_i[q][j] += pp[256 * RPT * NMAX + e];
+        }
+      }
+  }
+  rows_epilogue<CPLX, NMAX, RPT>(seg, it.tm * (256 * RPT), tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
 }  // namespace
@@ -418,16 +591,24 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
   const auto *B = static_cast<const double *>(b);
   auto *Cp = static_cast<double *>(c);
   // MFMA classes: 64x64, 32x64, 64x32, 32x32 (tiles stored class-major)
-  int64_t base = 0;
+  int64_t base = 0, red_base = 0;
   for (int cls = 0; cls < kNumMfmaClasses; ++cls) {
     const int64_t cnt = plan->cls_tiles[cls];
+    const int64_t n_red = plan->cls_red[cls];
     const Tile *tp = plan->d_tiles + base;
+    const ReduceItem *rp = plan->d_red + red_base;
     base += cnt;
+    red_base += n_red;
     if (!mfma_class || cnt == 0) continue;
     const dim3 grid(static_cast<unsigned>(cnt));
+    // k-sliced tiles: the slices write partial sums, then one reduction
+    // workgroup per sliced tile adds them and applies alpha / beta
 #define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
   hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, st, tp, plan->d_segs,       \
-                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi)
+                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi)
+#define QLT_LAUNCH_RED(CPLXV, TMV, TNV)                                                                   \
+  hipLaunchKernelGGL((k_reduce_slices<CPLXV, TMV, TNV>), dim3(static_cast<unsigned>(n_red)), block, 0, st,  \
+                     rp, plan->d_segs, plan->d_part, Cp, ar, ai, br, bi)
     probe_in(cls);
     if (dtype == QLT_F64) {
       if (cls == 0) QLT_LAUNCH_MFMA(false, 64, 64);
@@ -441,31 +622,50 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
       else QLT_LAUNCH_MFMA(true, 32, 32);
     }
     probe_out(cls);
+    if (n_red > 0) {
+      if (dtype == QLT_F64) {
+        if (cls == 0) QLT_LAUNCH_RED(false, 64, 64);
+        else if (cls == 1) QLT_LAUNCH_RED(false, 32, 64);
+        else if (cls == 2) QLT_LAUNCH_RED(false, 64, 32);
+        else QLT_LAUNCH_RED(false, 32, 32);
+      } else {
+        if (cls == 0) QLT_LAUNCH_RED(true, 64, 64);
+        else if (cls == 1) QLT_LAUNCH_RED(true, 32, 64);
+        else if (cls == 2) QLT_LAUNCH_RED(true, 64, 32);
+        else QLT_LAUNCH_RED(true, 32, 32);
+      }
+    }
+#undef QLT_LAUNCH_RED
 #undef QLT_LAUNCH_MFMA
   }
-  const int64_t n0 = base;
-  const int64_t n1 = plan->cls_tiles[kTileRows4], n2 = plan->cls_tiles[kTileRows16];
-  if (n1 > 0) {
-    const Tile *t1 = plan->d_tiles + n0;
-    probe_in(kTileRows4);
-    if (dtype == QLT_F64)
-      hipLaunchKernelGGL((k_rows_f64<false, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
-    else
-      hipLaunchKernelGGL((k_rows_f64<true, 4, 4>), dim3(static_cast<unsigned>(n1)), block, 0, st, t1,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
-    probe_out(kTileRows4);
-  }
-  if (n2 > 0) {
-    const Tile *t2 = plan->d_tiles + n0 + n1;
-    probe_in(kTileRows16);
-    if (dtype == QLT_F64)
-      hipLaunchKernelGGL((k_rows_f64<false, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
-    else
-      hipLaunchKernelGGL((k_rows_f64<true, 16, 1>), dim3(static_cast<unsigned>(n2)), block, 0, st, t2,
-                         plan->d_segs, plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, ar, ai, br, bi);
-    probe_out(kTileRows16);
+  // row classes (n <= 4, n <= 16), each followed by its k-slice reduction
+  for (int32_t cls = kTileRows4; cls <= kTileRows16; ++cls) {
+    const int64_t cnt = plan->cls_tiles[cls];
+    const int64_t n_red = plan->cls_red[cls];
+    const Tile *tp = plan->d_tiles + base;
+    const ReduceItem *rp = plan->d_red + red_base;
+    base += cnt;
+    red_base += n_red;
+    if (cnt == 0) continue;
+    const dim3 grid(static_cast<unsigned>(cnt)), rgrid(static_cast<unsigned>(n_red > 0 ? n_red : 1));
+#define QLT_LAUNCH_ROWS(CPLXV, NMAXV, RPTV)                                                                     \
+  do {                                                                                                          \
+    hipLaunchKernelGGL((k_rows_f64<CPLXV, NMAXV, RPTV>), grid, block, 0, st, tp, plan->d_segs, plan->d_exec,   \
+                       plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi);          \
+    probe_out(cls);                                                                                             \
+    if (n_red > 0)                                                                                              \
+      hipLaunchKernelGGL((k_reduce_rows<CPLXV, NMAXV, RPTV>), rgrid, block, 0, st, rp, plan->d_segs,           \
+                         plan->d_part, Cp, ar, ai, br, bi);                                                     \
+  } while (0)
+    probe_in(cls);
+    if (cls == kTileRows4) {
+      if (dtype == QLT_F64) QLT_LAUNCH_ROWS(false, 4, 4);
+      else QLT_LAUNCH_ROWS(true, 4, 4);
+    } else {
+      if (dtype == QLT_F64) QLT_LAUNCH_ROWS(false, 16, 1);
+      else QLT_LAUNCH_ROWS(true, 16, 1);
+    }
+#undef QLT_LAUNCH_ROWS
   }
   QLT_HIP(hipGetLastError());
   return QLT_OK;

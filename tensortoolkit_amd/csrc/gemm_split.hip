@@ -526,7 +526,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
       const Seg &sg = plan->h_segs[e.second];
       const int ntm = (sg.m + TM - 1) / TM, ntn = (sg.n + TN - 1) / TN;
       for (int a = 0; a < ntm; ++a)
-        for (int b = 0; b < ntn; ++b) L.tiles.push_back(Tile{e.second, a, b, 0});
+        for (int b = 0; b < ntn; ++b) L.tiles.push_back(Tile{e.second, a, b, 0, 0, 0, 0, -1});
     }
   }
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };

@@ -346,6 +346,10 @@ __global__ void k_tiles(const Seg *segs, const int64_t *tile_start, const int64_
     t.tm = static_cast<int32_t>(q / ntn);
     t.tn = static_cast<int32_t>(q % ntn);
     t.cls = s.cls;
+    t.t_begin = s.t0;
+    t.k_begin = 0;
+    t.n_chunks = 0;       // set on the host for the MFMA classes
+    t.part = -1;
     tiles[base + q] = t;
     // class-major, then longest first inside a class
     tkey[base + q] = (static_cast<uint64_t>(s.cls) << 40) | static_cast<uint64_t>(kmax - s.k_total);
@@ -443,6 +447,107 @@ struct DevPool {
   }
 };
 
+// k-slicing of long tiles (MFMA and row classes alike).  Per class, a step budget Q = max(kMinSliceChunks,
+// ceil(total steps / kSliceTargetWGs)) is set; a tile of w > Q steps is cut into
+// ceil(w / Q) (at most kMaxSlices) slices of nearly equal length along its
+// (task, k-chunk) sequence, each writing an unscaled partial sum that a
+// reduction kernel adds in slice order.  Classes whose tiles are many and even
+// stay unsliced; few or long tiles (a contraction over a long k onto a small
+// output) spread over the chip.  Slices of >= kMinSliceChunks steps keep the
+// partial-sum traffic small against the slice's MFMA work.
+constexpr int64_t kSliceTargetWGs = 2048;
+constexpr int64_t kMinSliceChunks = 8;
+constexpr int64_t kMaxSlices = 64;
+
+qlt_status slice_long_tiles(qlt_plan *plan) {
+  const int64_t n = plan->n_tiles;
+  if (n == 0) return QLT_OK;
+  std::vector<Tile> h(static_cast<size_t>(n));
+  QLT_HIP(hipMemcpy(h.data(), plan->d_tiles, sizeof(Tile) * n, hipMemcpyDeviceToHost));
+  const auto &segs = plan->h_segs;
+  const auto &ex = plan->h_exec;
+  auto steps_of = [](int64_t k) { return (k + kKChunk - 1) / kKChunk; };
+  std::vector<int64_t> seg_steps(segs.size(), 0);
+  for (size_t o = 0; o < segs.size(); ++o)
+    for (int32_t t = segs[o].t0; t < segs[o].t1; ++t) seg_steps[o] += steps_of(ex[static_cast<size_t>(t)].k);
+  std::vector<Tile> out;
+  std::vector<ReduceItem> red;
+  out.reserve(static_cast<size_t>(n));
+  int64_t base = 0, part_doubles = 0;
+  for (int32_t cls = 0; cls < kNumTileClasses; ++cls) {
+    const int64_t cnt = plan->cls_tiles[cls];
+    // partial-sum doubles of one slice (complex), in the kernels' thread order
+    const int64_t slice_doubles = is_mfma_class(cls) ? int64_t(mfma_tile_m(cls)) * mfma_tile_n(cls) * 2
+                                                     : int64_t(rows_per_tile(cls)) * (cls == kTileRows4 ? 4 : 16) * 2;
+    int64_t total = 0;
+    for (int64_t i = base; i < base + cnt; ++i) total += seg_steps[static_cast<size_t>(h[i].seg)];
+    const int64_t q = std::max(kMinSliceChunks, (total + kSliceTargetWGs - 1) / kSliceTargetWGs);
+    std::vector<Tile> cl;
+    int64_t parts = 0, n_red = 0;
+    for (int64_t i = base; i < base + cnt; ++i) {
+      Tile t = h[static_cast<size_t>(i)];
+      const Seg &s = segs[static_cast<size_t>(t.seg)];
+      const int64_t w = seg_steps[static_cast<size_t>(t.seg)];
+      const int64_t ns = std::min(kMaxSlices, (w + q - 1) / q);
+      t.t_begin = s.t0;
+      t.k_begin = 0;
+      t.n_chunks = static_cast<int32_t>(w);
+      t.part = -1;
+      if (ns <= 1) {
+        cl.push_back(t);
+        continue;
+      }
+      red.push_back(ReduceItem{t.seg, t.tm, t.tn, cls, static_cast<int32_t>(parts), static_cast<int32_t>(ns)});
+      ++n_red;
+      int32_t task = s.t0;
+      int64_t in_task = 0;     // steps already taken inside `task`
+      for (int64_t sl = 0; sl < ns; ++sl) {
+        const int64_t len = w / ns + (sl < w % ns ? 1 : 0);
+        Tile u = t;
+        u.t_begin = task;
+        u.k_begin = static_cast<int32_t>(in_task * kKChunk);
+        u.n_chunks = static_cast<int32_t>(len);
+        u.part = static_cast<int32_t>(parts + sl);
+        cl.push_back(u);
+        for (int64_t rem = len; rem > 0;) {
+          const int64_t left = steps_of(ex[static_cast<size_t>(task)].k) - in_task;
+          if (rem < left) {
+            in_task += rem;
+            rem = 0;
+          } else {
+            rem -= left;
+            ++task;
+            in_task = 0;
+          }
+        }
+      }
+      parts += ns;
+    }
+    std::stable_sort(cl.begin(), cl.end(), [](const Tile &a, const Tile &b) { return a.n_chunks > b.n_chunks; });
+    out.insert(out.end(), cl.begin(), cl.end());
+    plan->cls_tiles[cls] = static_cast<int64_t>(cl.size());
+    plan->cls_red[cls] = n_red;
+    part_doubles = std::max(part_doubles, parts * slice_doubles);
+    base += cnt;
+  }
+  QLT_CHECK_ARG(static_cast<int64_t>(out.size()) < (int64_t(1) << 31), "qlt_plan_build: too many tiles after k-slicing");
+  if (static_cast<int64_t>(out.size()) != n) {
+    Tile *nt = nullptr;
+    QLT_HIP(hipMalloc(&nt, sizeof(Tile) * out.size()));
+    QLT_HIP(hipFree(plan->d_tiles));
+    plan->d_tiles = nt;
+  }
+  QLT_HIP(hipMemcpy(plan->d_tiles, out.data(), sizeof(Tile) * out.size(), hipMemcpyHostToDevice));
+  plan->n_tiles = static_cast<int64_t>(out.size());
+  if (!red.empty()) {
+    QLT_HIP(hipMalloc(&plan->d_red, sizeof(ReduceItem) * red.size()));
+    QLT_HIP(hipMemcpy(plan->d_red, red.data(), sizeof(ReduceItem) * red.size(), hipMemcpyHostToDevice));
+    QLT_HIP(hipMalloc(&plan->d_part, sizeof(double) * static_cast<size_t>(part_doubles)));
+    plan->part_doubles = static_cast<size_t>(part_doubles);
+  }
+  return QLT_OK;
+}
+
 void free_plan_device(qlt_plan *p) {
   if (p->d_exec) (void)hipFree(p->d_exec);
   if (p->d_segs) (void)hipFree(p->d_segs);
@@ -450,6 +555,10 @@ void free_plan_device(qlt_plan *p) {
   if (p->d_ga) (void)hipFree(p->d_ga);
   if (p->d_gb) (void)hipFree(p->d_gb);
   if (p->d_work) (void)hipFree(p->d_work);
+  if (p->d_red) (void)hipFree(p->d_red);
+  if (p->d_part) (void)hipFree(p->d_part);
+  p->d_red = nullptr;
+  p->d_part = nullptr;
   p->d_exec = nullptr;
   p->d_segs = nullptr;
   p->d_tiles = nullptr;
@@ -886,6 +995,7 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
     }
     plan->sum_mkn = sum;
   }
+  QLT_TRY(slice_long_tiles(plan.get()));
   *plan_out = plan.release();
   return QLT_OK;
 }

@@ -121,12 +121,35 @@ __host__ __device__ constexpr int64_t tiles_of(int32_t cls, int64_t m, int64_t n
              : (m + rows_per_tile(cls) - 1) / rows_per_tile(cls);
 }
 
-// One output tile of the grouped kernel.
+// k extent of one pipeline step of the MFMA kernels; a task of contracted
+// extent k takes ceil(k / kKChunk) steps.
+constexpr int32_t kKChunk = 16;
+
+// One output tile of the grouped kernel, or one k-slice of it.  A k-slice
+// covers `n_chunks` consecutive steps of the output block's (task, k-chunk)
+// sequence starting at task `t_begin`, k offset `k_begin`, and writes its
+// unscaled partial sum to partial buffer `part`; unsliced tiles have part = -1
+// and cover the whole sequence.
 struct Tile {
   int32_t seg;
   int32_t tm;      // tile row index within the block
   int32_t tn;      // tile column index within the block
   int32_t cls;     // tile-size class
+  int32_t t_begin;
+  int32_t k_begin;
+  int32_t n_chunks;
+  int32_t part;
+};
+
+// Output tile whose k-slices are summed by the reduction kernel: partial
+// buffers part0 .. part0 + n_parts - 1, in slice order.
+struct ReduceItem {
+  int32_t seg;
+  int32_t tm;
+  int32_t tn;
+  int32_t cls;
+  int32_t part0;
+  int32_t n_parts;
 };
 
 // Axis-split layout of a plan (same permutation for every block pair).
@@ -184,6 +207,11 @@ struct qlt_plan {
   int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};  // tiles per class, class-major in d_tiles
   double cls_mkn[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};     // sum of m*k*n over the class's tasks
   double cls_elems[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};   // distinct A + B block elements + C elements
+  // ---- k-slicing of long MFMA tiles (native path) ----
+  qlt::ReduceItem *d_red = nullptr;     // class-major reduction items
+  int64_t cls_red[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};
+  double *d_part = nullptr;             // k-slice partial sums, sized for complex128, shared by the classes
+  size_t part_doubles = 0;
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;
@@ -197,7 +225,8 @@ struct qlt_plan {
   qlt_plan &operator=(const qlt_plan &) = delete;
   ~qlt_plan() {
     for (void *p : {static_cast<void *>(d_exec), static_cast<void *>(d_segs), static_cast<void *>(d_tiles),
-                    static_cast<void *>(d_ga), static_cast<void *>(d_gb), d_work})
+                    static_cast<void *>(d_ga), static_cast<void *>(d_gb), d_work, static_cast<void *>(d_red),
+                    static_cast<void *>(d_part)})
       if (p != nullptr) (void)hipFree(p);
   }
 };
