@@ -43,20 +43,6 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return (r / kXcdRun) * span + (b % kXcds) * kXcdRun + (r % kXcdRun);
 }
 
-// Offset of linear index `x` inside a block, decomposed row-major over
-// dims[0..n-1] (last fastest) with the given memory strides.
-__device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, const int32_t *strides,
-                                                int n) {
-  int32_t off = 0;
-  for (int j = n - 1; j >= 0; --j) {
-    const int32_t d = dims[j];
-    const int32_t q = x / d;
-    off += (x - q * d) * strides[j];
-    x = q;
-  }
-  return off;
-}
-
 // Grouped MFMA kernel, one 256-thread workgroup per TM x TN output tile
 // (TM, TN in {32, 64}); the four waves split the tile 2 x 2, each wave
 // computing (TM/2) x (TN/2) as BM x BN blocks of v_mfma_f64_16x16x4f64.
@@ -107,27 +93,29 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   int32_t a_row[QA], b_col[QB];
   bool a_plain = false, b_plain = false;
   double2 pa[QA], pb[QB];
+  // every task of this output block reads plain blocks: no descriptor reads
+  const bool seg_plain = (seg.flags & (kSegAPlain | kSegBPlain)) == (kSegAPlain | kSegBPlain);
   auto task_rows = [&](const BlkGather &gA, const BlkGather &gB, const ExecTask &et) {
-    a_plain = gA.plain != 0;
-    b_plain = gB.plain != 0;
-    akf = a_plain || (L.a_kfast != 0);
-    bnf = b_plain || (L.b_nfast != 0);
+    a_plain = seg_plain || gA.plain != 0;
+    b_plain = seg_plain || gB.plain != 0;
+    akf = a_plain || gA.fastcol != 0;
+    bnf = b_plain || gB.fastcol != 0;
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int r = i0 + a_r(q);
-      a_row[q] = (r < seg.m) ? (a_plain ? r * et.k : split_offset(r, gA.rdim, gA.rstr, L.a_nr)) : -1;
+      a_row[q] = (r < seg.m) ? (a_plain ? r * et.k : split_offset(r, gA.rdim, gA.rstr, gA.rmag, gA.rsh, L.a_nr)) : -1;
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int c = j0 + b_c(q);
-      b_col[q] = (c < seg.n) ? (b_plain ? c : split_offset(c, gB.cdim, gB.cstr, L.b_nc)) : -1;
+      b_col[q] = (c < seg.n) ? (b_plain ? c : split_offset(c, gB.cdim, gB.cstr, gB.cmag, gB.csh, L.b_nc)) : -1;
     }
   };
   auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int pk = kk + a_p(q);
-      const int32_t ac = (pk < et.k) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, L.a_nc)) : -1;
+      const int32_t ac = (pk < et.k) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
       const bool ok = (a_row[q] >= 0) && (ac >= 0);
       const int64_t e = et.a_off + a_row[q] + ac;
       if (CPLX)
@@ -138,7 +126,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int pk = kk + b_p(q);
-      const int32_t br = (pk < et.k) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, L.b_nr)) : -1;
+      const int32_t br = (pk < et.k) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
       const bool ok = (b_col[q] >= 0) && (br >= 0);
       const int64_t e = et.b_off + br + b_col[q];
       if (CPLX)
@@ -425,20 +413,20 @@ __global__ __launch_bounds__(256) void k_rows_f64(
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int row = r0 + tid + 256 * q;
-        roff[q] = (row < seg.m) ? (a_plain ? row * et.k : split_offset(row, gA.rdim, gA.rstr, L.a_nr)) : -1;
+        roff[q] = (row < seg.m) ? (a_plain ? row * et.k : split_offset(row, gA.rdim, gA.rstr, gA.rmag, gA.rsh, L.a_nr)) : -1;
       }
       __syncthreads();
       if (tid < NMAX)
-        s_bcol[tid] = (tid < n) ? (b_plain ? tid : split_offset(tid, gB.cdim, gB.cstr, L.b_nc)) : -1;
+        s_bcol[tid] = (tid < n) ? (b_plain ? tid : split_offset(tid, gB.cdim, gB.cstr, gB.cmag, gB.csh, L.b_nc)) : -1;
     }
     const BlkGather &gA = ga[et.a_blk];
     const BlkGather &gB = gb[et.b_blk];
     if (tid < RK) {
       const int p = kk + tid;
-      s_acol[tid] = (p < et.k) ? (a_plain ? p : split_offset(p, gA.cdim, gA.cstr, L.a_nc)) : -1;
+      s_acol[tid] = (p < et.k) ? (a_plain ? p : split_offset(p, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
     } else if (tid >= 64 && tid < 64 + RK) {
       const int p = kk + tid - 64;
-      s_brow[tid - 64] = (p < et.k) ? (b_plain ? p * n : split_offset(p, gB.rdim, gB.rstr, L.b_nr)) : -1;
+      s_brow[tid - 64] = (p < et.k) ? (b_plain ? p * n : split_offset(p, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
     }
     __syncthreads();
     if (tid < RK * NMAX) {

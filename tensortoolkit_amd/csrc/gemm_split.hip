@@ -66,18 +66,6 @@ struct PackItem {
   int32_t pad;
 };
 
-__device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, const int32_t *strides,
-                                                int n) {
-  int32_t off = 0;
-  for (int j = n - 1; j >= 0; --j) {
-    const int32_t d = dims[j];
-    const int32_t q = x / d;
-    off += (x - q * d) * strides[j];
-    x = q;
-  }
-  return off;
-}
-
 __device__ __forceinline__ uint16_t bf16_bits_exact(double v) {
   // v is c * 2^-m with |c| <= 256 an integer: exactly representable in BF16,
   // so truncating the FP32 bit pattern is exact.
@@ -116,16 +104,20 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   const int32_t *rstr = is_a ? g.rstr : g.cstr;
   const int32_t *cdim = is_a ? g.cdim : g.rdim;
   const int32_t *cstr = is_a ? g.cstr : g.rstr;
+  const uint64_t *rmag = is_a ? g.rmag : g.cmag;
+  const int32_t *rsh = is_a ? g.rsh : g.csh;
+  const uint64_t *cmag = is_a ? g.cmag : g.rmag;
+  const int32_t *csh = is_a ? g.csh : g.rsh;
   const int nr = is_a ? L.a_nr : L.b_nc;
   const int nc = is_a ? L.a_nc : L.b_nr;
   // A plain block: row r, col p at r * cols + p (k contiguous).  A plain B
   // block seen transposed: row j, col p at p * rows + j (rows contiguous).
   const bool plain = g.plain != 0;
   // whether consecutive k are consecutive in memory
-  const bool kfast = plain ? is_a : (is_a ? (L.a_kfast != 0) : (L.b_nfast == 0));
+  const bool kfast = plain ? is_a : (is_a ? (g.fastcol != 0) : (g.fastcol == 0));
   const int tid = threadIdx.x;
-  auto row_off = [&](int r) { return plain ? (is_a ? r * pm.cols : r) : split_offset(r, rdim, rstr, nr); };
-  auto col_off = [&](int p) { return plain ? (is_a ? p : p * pm.rows) : split_offset(p, cdim, cstr, nc); };
+  auto row_off = [&](int r) { return plain ? (is_a ? r * pm.cols : r) : split_offset(r, rdim, rstr, rmag, rsh, nr); };
+  auto col_off = [&](int p) { return plain ? (is_a ? p : p * pm.rows) : split_offset(p, cdim, cstr, cmag, csh, nc); };
 
   if (tid < 64) {
     const int r = it.r0 + tid;

@@ -326,7 +326,7 @@ __global__ void k_segs(const int64_t *c_first, const int64_t *c_off, const int32
   for (int32_t t = s.t0; t < s.t1; ++t) kt += exec[t].k;
   s.k_total = kt;
   s.cls = tile_class_of(m, n);
-  s.pad = 0;
+  s.flags = 0;      // plain flags are set on the host once the gather descriptors exist
   segs[o] = s;
   tile_cnt[o] = tiles_of(s.cls, m, n);
   cost[o] = kt;
@@ -404,7 +404,22 @@ __global__ void k_gather_desc(const int32_t *dims, int64_t n, const PlanArgs arg
   g.plain = plain;
   g.ncols = ncols;
   g.nrows = nrows;
-  g.pad = 0;
+  // fastcol: the block's innermost axis of extent > 1 (stride 1) is one of
+  // the col axes, so walking the col index walks memory contiguously
+  int32_t inner = -1;
+  for (int j = rank - 1; j >= 0; --j)
+    if (d[j] > 1) {
+      inner = j;
+      break;
+    }
+  int32_t fastcol = 0;
+  for (int j = 0; j < nc; ++j)
+    if (caxes[j] == inner) fastcol = 1;
+  g.fastcol = fastcol;
+  for (int q = 0; q < QLT_MAX_RANK; ++q) {
+    magic_of(static_cast<uint32_t>(g.rdim[q]), g.rmag[q], g.rsh[q]);
+    magic_of(static_cast<uint32_t>(g.cdim[q]), g.cmag[q], g.csh[q]);
+  }
   out[i] = g;
 }
 
@@ -446,6 +461,26 @@ struct DevPool {
     return QLT_OK;
   }
 };
+
+// Marks the output blocks whose every task reads a plain A block and/or a
+// plain B block; the MFMA kernel runs those on its division-free path.
+qlt_status mark_plain_segments(qlt_plan *plan) {
+  if (plan->n_segs == 0) return QLT_OK;
+  std::vector<BlkGather> ga(static_cast<size_t>(plan->n_a_blks)), gb(static_cast<size_t>(plan->n_b_blks));
+  QLT_HIP(hipMemcpy(ga.data(), plan->d_ga, sizeof(BlkGather) * ga.size(), hipMemcpyDeviceToHost));
+  QLT_HIP(hipMemcpy(gb.data(), plan->d_gb, sizeof(BlkGather) * gb.size(), hipMemcpyDeviceToHost));
+  for (Seg &sg : plan->h_segs) {
+    int32_t f = kSegAPlain | kSegBPlain;
+    for (int32_t t = sg.t0; t < sg.t1; ++t) {
+      const ExecTask &e = plan->h_exec[static_cast<size_t>(t)];
+      if (!ga[static_cast<size_t>(e.a_blk)].plain) f &= ~kSegAPlain;
+      if (!gb[static_cast<size_t>(e.b_blk)].plain) f &= ~kSegBPlain;
+    }
+    sg.flags = f;
+  }
+  QLT_HIP(hipMemcpy(plan->d_segs, plan->h_segs.data(), sizeof(Seg) * plan->h_segs.size(), hipMemcpyHostToDevice));
+  return QLT_OK;
+}
 
 // k-slicing of long tiles (MFMA and row classes alike).  Per class, a step budget Q = max(kMinSliceChunks,
 // ceil(total steps / kSliceTargetWGs)) is set; a tile of w > Q steps is cut into
@@ -995,6 +1030,7 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
     }
     plan->sum_mkn = sum;
   }
+  QLT_TRY(mark_plain_segments(plan.get()));
   QLT_TRY(slice_long_tiles(plan.get()));
   *plan_out = plan.release();
   return QLT_OK;

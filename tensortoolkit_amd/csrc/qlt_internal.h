@@ -54,7 +54,14 @@ bool stream_capturing(hipStream_t s);
 // (row axes ++ col axes), each part decomposed row-major, last axis fastest.
 // For A the rows are the saved axes and the cols the contracted axes; for B
 // the rows are the contracted axes and the cols the saved axes.
+// Each dim also carries a multiply-shift reciprocal: for 0 <= x < 2^31,
+// x / dim == (x * mag) >> sh with sh = 31 + ceil(log2 dim) and
+// mag = ceil(2^sh / dim) (< 2^33).
 struct BlkGather {
+  uint64_t rmag[QLT_MAX_RANK];
+  uint64_t cmag[QLT_MAX_RANK];
+  int32_t rsh[QLT_MAX_RANK];
+  int32_t csh[QLT_MAX_RANK];
   int32_t rdim[QLT_MAX_RANK];
   int32_t rstr[QLT_MAX_RANK];
   int32_t cdim[QLT_MAX_RANK];
@@ -65,7 +72,7 @@ struct BlkGather {
   int32_t plain;
   int32_t ncols;   // product of cdim (the leading dimension when plain)
   int32_t nrows;   // product of rdim
-  int32_t pad;
+  int32_t fastcol; // 1 when the block's innermost axis of extent > 1 is a col axis
 };
 
 // One task in execution order (sorted by output block, generation order
@@ -88,8 +95,32 @@ struct Seg {
   int32_t t1;      // one past the last ExecTask
   int64_t k_total; // sum of k over the segment's tasks
   int32_t cls;     // tile class of the segment (kTileMfma, kTileRows4, kTileRows16)
-  int32_t pad;
+  int32_t flags;   // kSegAPlain | kSegBPlain when every task's A (B) block is plain
 };
+constexpr int32_t kSegAPlain = 1;
+constexpr int32_t kSegBPlain = 2;
+
+// Offset of linear index `x` (0 <= x < 2^31) inside a block, decomposed
+// row-major over dims[0..n-1] (last fastest) with the given memory strides;
+// the divisions use the dims' multiply-shift reciprocals.
+__device__ __forceinline__ int32_t split_offset(int32_t x, const int32_t *dims, const int32_t *strides,
+                                                const uint64_t *mags, const int32_t *shs, int n) {
+  int32_t off = 0;
+  for (int j = n - 1; j >= 0; --j) {
+    const int32_t q = static_cast<int32_t>((static_cast<uint64_t>(x) * mags[j]) >> shs[j]);
+    off += (x - q * dims[j]) * strides[j];
+    x = q;
+  }
+  return off;
+}
+
+// Multiply-shift reciprocal of d (1 <= d < 2^31), exact for 0 <= x < 2^31.
+__host__ __device__ inline void magic_of(uint32_t d, uint64_t &mag, int32_t &sh) {
+  int32_t l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  sh = 31 + l;
+  mag = ((uint64_t(1) << sh) + d - 1) / d;
+}
 
 // Tile classes.  Classes 0..3: MFMA output tiles of 64x64, 32x64, 64x32 and
 // 32x32 (rows x cols), chosen per output block to limit padding.
