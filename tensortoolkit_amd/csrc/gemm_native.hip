@@ -25,7 +25,8 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int KC = kKChunk;   // k-chunk staged through LDS per pipeline step
+constexpr int KC = kKChunk;   // plan step: k-chunk of 16
+constexpr int kRealUnits = 1; // plan steps staged per pipeline step (FP64); 2 halves the occupancy and measured slower
 
 // Tile order for the XCDs: workgroups are dispatched to the 8 XCDs round
 // robin, so workgroup b runs on XCD b % 8.  Mapping b to logical tile
@@ -58,10 +59,15 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
-  constexpr int QA = TMv / 16, QB = TNv / 16;          // elements per thread per chunk
+  // k extent per pipeline step: U plan steps of 16 (FP64 stages 32 k per
+  // step; complex128, whose LDS tiles are twice as large, stages 16)
+  constexpr int U = CPLX ? 1 : kRealUnits;
+  constexpr int KCv = KC * U;
+  constexpr int QA0 = TMv / 16, QB0 = TNv / 16;        // elements per thread per 16 k
+  constexpr int QA = QA0 * U, QB = QB0 * U;            // elements per thread per step
   constexpr int BM = TMv / 32, BN = TNv / 32;          // MFMA blocks per wave
-  __shared__ double As[2][P][KC][TMv + 1];
-  __shared__ double Bs[2][P][KC][TNv + 1];
+  __shared__ double As[2][P][KCv][TMv + 1];
+  __shared__ double Bs[2][P][KCv][TNv + 1];
 
   const Tile tile = tiles[xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const Seg seg = segs[tile.seg];
@@ -82,10 +88,10 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       acc_i[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  auto a_r = [&](int q) { return akf ? (tid >> 4) + 16 * q : tid % TMv; };
-  auto a_p = [&](int q) { return akf ? (tid & 15) : tid / TMv + (256 / TMv) * q; };
-  auto b_c = [&](int q) { return bnf ? tid % TNv : (tid >> 4) + 16 * q; };
-  auto b_p = [&](int q) { return bnf ? tid / TNv + (256 / TNv) * q : (tid & 15); };
+  auto a_r = [&](int q) { return akf ? (tid >> 4) + 16 * (q % QA0) : tid % TMv; };
+  auto a_p = [&](int q) { return akf ? (tid & 15) + 16 * (q / QA0) : tid / TMv + (256 / TMv) * q; };
+  auto b_c = [&](int q) { return bnf ? tid % TNv : (tid >> 4) + 16 * (q % QB0); };
+  auto b_p = [&](int q) { return bnf ? tid / TNv + (256 / TNv) * q : (tid & 15) + 16 * (q / QB0); };
 
   // Per task: the row offsets of the A' elements and column offsets of the
   // B' elements this thread loads.  A block whose permuted view is plain
@@ -111,11 +117,11 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       b_col[q] = (c < seg.n) ? (b_plain ? c : split_offset(c, gB.cdim, gB.cstr, gB.cmag, gB.csh, L.b_nc)) : -1;
     }
   };
-  auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk) {
+  auto fetch = [&](const ExecTask &et, const BlkGather &gA, const BlkGather &gB, int kk, int kend) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int pk = kk + a_p(q);
-      const int32_t ac = (pk < et.k) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
+      const int32_t ac = (pk < kend) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
       const bool ok = (a_row[q] >= 0) && (ac >= 0);
       const int64_t e = et.a_off + a_row[q] + ac;
       if (CPLX)
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int pk = kk + b_p(q);
-      const int32_t br = (pk < et.k) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
+      const int32_t br = (pk < kend) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
       const bool ok = (b_col[q] >= 0) && (br >= 0);
       const int64_t e = et.b_off + br + b_col[q];
       if (CPLX)
@@ -149,30 +155,39 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   };
 
   // ---- pipeline over this tile's steps of the (task, k-chunk) sequence ----
-  int t = tile.t_begin, kk = tile.k_begin;
+  // A step stages up to U plan steps of 16 k, never crossing a task or the
+  // end of the tile's slice; k beyond the step's end loads as zero.
+  auto span_of = [&](const ExecTask &e, int kk_, int left_) {
+    return min(U, min((e.k - kk_ + KC - 1) / KC, left_));
+  };
+  int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
   ExecTask et = exec[t];
   task_rows(ga[et.a_blk], gb[et.b_blk], et);
-  fetch(et, ga[et.a_blk], gb[et.b_blk], kk);
+  int sp = span_of(et, kk, left);
+  fetch(et, ga[et.a_blk], gb[et.b_blk], kk, min(et.k, kk + KC * sp));
   stage(0);
   __syncthreads();
   int buf = 0;
-  for (int step = 0; step < tile.n_chunks; ++step) {
-    int t2 = t, kk2 = kk + KC;
+  while (left > 0) {
+    const int left2 = left - sp;
+    int t2 = t, kk2 = kk + KC * sp;
     if (kk2 >= et.k) {
       t2 = t + 1;
       kk2 = 0;
     }
-    const bool has_next = step + 1 < tile.n_chunks;
+    const bool has_next = left2 > 0;
     ExecTask et2 = et;
+    int sp2 = 0;
     if (has_next) {
       if (t2 != t) {
         et2 = exec[t2];
         task_rows(ga[et2.a_blk], gb[et2.b_blk], et2);
       }
-      fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2);
+      sp2 = span_of(et2, kk2, left2);
+      fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2, min(et2.k, kk2 + KC * sp2));
     }
 #pragma unroll
-    for (int ks = 0; ks < KC; ks += 4) {
+    for (int ks = 0; ks < KCv; ks += 4) {
       double a[BM], b[BN];
 #pragma unroll
       for (int mt = 0; mt < BM; ++mt) a[mt] = As[buf][0][ks + lk][wm + mt * 16 + lr];
@@ -205,6 +220,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     t = t2;
     kk = kk2;
     et = et2;
+    left = left2;
+    sp = sp2;
     buf ^= 1;
   }
 
