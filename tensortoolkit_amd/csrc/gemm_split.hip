@@ -450,6 +450,181 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   }
 }
 
+// Complex split GEMM with the output components on separate waves: one
+// 512-thread workgroup per 64 x 64 output tile; waves 0-3 accumulate the real
+// part (ar*br - ai*bi) and waves 4-7 the imaginary part (ar*bi + ai*br) of the
+// same 2 x 2 arrangement of 32 x 32 blocks, each over all 15 slice products,
+// with the same FP32 groups and flush points as k_split_gemm (so the same
+// exactness argument holds).  Halving the accumulators per wave lets two waves
+// share each SIMD, which hides the load and barrier latency that a single
+// wave per SIMD exposes.  The imaginary halves reach the real-part waves
+// through LDS for the epilogue.
+__global__ __launch_bounds__(512) void k_split_gemm_c2(
+    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
+    const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
+    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int PARTS = 2;
+  constexpr int KCH = 32;                          // k between flushes (keeps levels 2..4 exact)
+  constexpr int CHUNKS = PARTS * S * 128;          // 16-byte pieces per operand per k-chunk
+  constexpr int NT = 512;
+  constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][PARTS][S][TM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
+  __shared__ int32_t s_ea[2][TM];
+  static_assert(sizeof(As) >= 4 * 16 * 64 * sizeof(double), "LDS exchange of the imaginary parts");
+
+  const Tile tile = tiles[blockIdx.x];
+  const Seg seg = segs[tile.seg];
+  const int i0 = tile.tm * TM, j0 = tile.tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int ws = w & 3, comp = w >> 2;             // spatial block, output component
+  const int wm = (ws >> 1) * 32, wn = (ws & 1) * 32;
+  const int lrow = lane & 31, lk = (lane >> 5) * 8;
+
+  double acc64[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+  v16f g[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) g[q] = v16f{};
+
+  v4u pa[PER_THREAD], pb[PER_THREAD];
+  int32_t ea_next = 0;
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
+    const int nkc = ma.k_pad / KC;
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + NT * u;
+      if (c < CHUNKS) {
+        const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+        const int64_t ia = ma.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * ma.r_pad + i0 + rr) * KC + half * 8;
+        const int64_t ib = mb.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * mb.r_pad + j0 + rr) * KC + half * 8;
+        pa[u] = *reinterpret_cast<const v4u *>(planes + ia);
+        pb[u] = *reinterpret_cast<const v4u *>(planes + ib);
+      }
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + NT * u;
+      if (c < CHUNKS) {
+        const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+        *reinterpret_cast<v4u *>(&As[buf][ps / S][ps % S][rr][half * 8]) = pa[u];
+        *reinterpret_cast<v4u *>(&Bs[buf][ps / S][ps % S][rr][half * 8]) = pb[u];
+      }
+    }
+  };
+
+  int t = seg.t0, kc = 0;
+  PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
+  int nkc = ma.k_pad / KC;
+  int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  fetch(ma, mb, 0);
+  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + i0 + tid];
+  stage(0);
+  __syncthreads();
+  int buf = 0, since_flush = 0;
+
+  while (t < seg.t1) {
+    int t2 = t, kc2 = kc + 1;
+    if (kc2 == nkc) {
+      t2 = t + 1;
+      kc2 = 0;
+    }
+    const bool has_next = t2 < seg.t1;
+    PackMeta ma2 = ma, mb2 = mb;
+    int32_t eb2 = eb;
+    if (has_next) {
+      if (t2 != t) {
+        const ExecTask e2 = exec[t2];
+        ma2 = meta_a[e2.a_blk];
+        mb2 = meta_b[e2.b_blk];
+        eb2 = exps[mb2.exp_off + j0 + wn + lrow];
+        if (tid < TM) ea_next = exps[ma2.exp_off + i0 + tid];
+      }
+      fetch(ma2, mb2, kc2);
+    }
+    {
+      // real part: ar*br + (-ai)*bi; imaginary part: ar*bi + ai*br
+      v8bf b0[S], b1[S];
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2) {
+        b0[s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][comp][s2][wn + lrow][lk]);
+        b1[s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][1 - comp][s2][wn + lrow][lk]);
+      }
+#pragma unroll
+      for (int sa = 0; sa < S; ++sa) {
+        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][0][sa][wm + lrow][lk]);
+        v8bf ai = *reinterpret_cast<const v8bf *>(&As[buf][1][sa][wm + lrow][lk]);
+        if (comp == 0) ai = neg8(ai);
+#pragma unroll
+        for (int sb = 0; sb < S - sa; ++sb) {
+          const int q = group_of(sa, sb);
+          g[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, b0[sb], g[q], 0, 0, 0);
+          g[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, b1[sb], g[q], 0, 0, 0);
+        }
+      }
+    }
+    since_flush += KC;
+    if (since_flush >= KCH || kc == nkc - 1) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int e = s_ea[buf][row] + eb - 16;
+        const double sum = static_cast<double>(g[0][r]) + static_cast<double>(g[1][r]) +
+                           static_cast<double>(g[2][r]) + static_cast<double>(g[3][r]);
+        acc64[r] += ldexp(sum, e);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[q] = v16f{};
+      since_flush = 0;
+    }
+    if (has_next) {
+      stage(buf ^ 1);
+      if (tid < TM) s_ea[buf ^ 1][tid] = (t2 != t) ? ea_next : s_ea[buf][tid];
+    }
+    __syncthreads();
+    if (t2 != t) {
+      ma = ma2;
+      mb = mb2;
+      nkc = ma.k_pad / KC;
+      eb = eb2;
+    }
+    t = t2;
+    kc = kc2;
+    buf ^= 1;
+  }
+
+  // ---- epilogue: imaginary parts to the real-part waves through LDS ----
+  double *xch = reinterpret_cast<double *>(&As[0][0][0][0][0]);
+  if (comp == 1) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xch[(ws * 16 + r) * 64 + lane] = acc64[r];
+  }
+  __syncthreads();
+  if (comp == 1) return;
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  const int col = j0 + wn + lrow;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (row < seg.m && col < seg.n) {
+      const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
+      const double xr = acc64[r], xi = xch[(ws * 16 + r) * 64 + lane];
+      double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+      double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+      if (use_beta) {
+        const double2 o = *cp;
+        yr += beta_r * o.x - beta_i * o.y;
+        yi += beta_r * o.y + beta_i * o.x;
+      }
+      *cp = double2{yr, yi};
+    }
+  }
+}
+
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 // Host-side layout of the split workspace of a plan.
@@ -672,14 +847,14 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
     if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_gemm<true>, dim3(n_mfma), blk, 0, st,
+    hipLaunchKernelGGL(k_split_gemm_c2, dim3(n_mfma), dim3(512), 0, st,
                        stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
                        static_cast<double *>(c), ar, ai, br, bi);
   }
   if (probe_split) {
     double elems = 0.0;
     for (int32_t cls = 0; cls < kNumMfmaClasses; ++cls) elems += plan->cls_elems[cls];
-    probe_end(st, cplx ? "k_split_gemm c128" : "k_split_gemm f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
+    probe_end(st, cplx ? "k_split_gemm_c2 c128" : "k_split_gemm f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
               (cplx ? 16.0 : 8.0) * elems);
   }
   QLT_HIP(hipGetLastError());
