@@ -154,38 +154,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     }
   };
 
-  // ---- pipeline over this tile's steps of the (task, k-chunk) sequence ----
-  // A step stages up to U plan steps of 16 k, never crossing a task or the
-  // end of the tile's slice; k beyond the step's end loads as zero.
-  auto span_of = [&](const ExecTask &e, int kk_, int left_) {
-    return min(U, min((e.k - kk_ + KC - 1) / KC, left_));
-  };
-  int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
-  ExecTask et = exec[t];
-  task_rows(ga[et.a_blk], gb[et.b_blk], et);
-  int sp = span_of(et, kk, left);
-  fetch(et, ga[et.a_blk], gb[et.b_blk], kk, min(et.k, kk + KC * sp));
-  stage(0);
-  __syncthreads();
-  int buf = 0;
-  while (left > 0) {
-    const int left2 = left - sp;
-    int t2 = t, kk2 = kk + KC * sp;
-    if (kk2 >= et.k) {
-      t2 = t + 1;
-      kk2 = 0;
-    }
-    const bool has_next = left2 > 0;
-    ExecTask et2 = et;
-    int sp2 = 0;
-    if (has_next) {
-      if (t2 != t) {
-        et2 = exec[t2];
-        task_rows(ga[et2.a_blk], gb[et2.b_blk], et2);
-      }
-      sp2 = span_of(et2, kk2, left2);
-      fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2, min(et2.k, kk2 + KC * sp2));
-    }
+  // ---- MFMAs on one staged step ----
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < KCv; ks += 4) {
       double a[BM], b[BN];
@@ -215,14 +185,141 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
           }
       }
     }
-    if (has_next) stage(buf ^ 1);
+  };
+
+  // ---- pipeline over this tile's steps of the (task, k-chunk) sequence ----
+  // A step stages up to U plan steps of 16 k, never crossing a task or the
+  // end of the tile's slice; k beyond the step's end loads as zero.
+  auto span_of = [&](const ExecTask &e, int kk_, int left_) {
+    return min(U, min((e.k - kk_ + KC - 1) / KC, left_));
+  };
+
+  if (seg_plain && U == 1) {
+    // Output block whose every A and B block is plain: A' element (r, p) at
+    // a_off + r*k + p, B' element (p, c) at b_off + p*n + c.  Each thread
+    // keeps one pointer per element it loads, set once per task and advanced
+    // by 16 columns (A) / 16 rows (B) per step; row and column bounds are
+    // fixed for the tile, and the k bound only matters on a task's last,
+    // partial step.
+    const int ap = tid & 15, ar0 = tid >> 4;
+    const int bc = tid % TNv, bp0 = tid / TNv;
+    constexpr int BPS = 256 / TNv;                     // B' rows between a thread's elements
+    bool aok[QA], bok[QB];
+    const double *aptr[QA], *bptr[QB];
+#pragma unroll
+    for (int q = 0; q < QA; ++q) aok[q] = i0 + ar0 + 16 * q < seg.m;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) bok[q] = j0 + bc < seg.n;
+    auto setup = [&](const ExecTask &e, int kk0) {
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        const int64_t r = aok[q] ? i0 + ar0 + 16 * q : 0;
+        aptr[q] = A + (e.a_off + r * e.k + ap + kk0) * P;
+      }
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const int64_t c = bok[q] ? j0 + bc : 0;
+        bptr[q] = B + (e.b_off + static_cast<int64_t>(bp0 + BPS * q + kk0) * seg.n + c) * P;
+      }
+    };
+    auto pfetch = [&](int kvalid) {
+      const bool full = kvalid >= KC;
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        const bool ok = aok[q] && (full || ap < kvalid);
+        if (CPLX) pa[q] = ok ? *reinterpret_cast<const double2 *>(aptr[q]) : double2{0.0, 0.0};
+        else pa[q] = double2{ok ? *aptr[q] : 0.0, 0.0};
+        aptr[q] += KC * P;
+      }
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const bool ok = bok[q] && (full || bp0 + BPS * q < kvalid);
+        if (CPLX) pb[q] = ok ? *reinterpret_cast<const double2 *>(bptr[q]) : double2{0.0, 0.0};
+        else pb[q] = double2{ok ? *bptr[q] : 0.0, 0.0};
+        bptr[q] += static_cast<int64_t>(KC) * seg.n * P;
+      }
+    };
+    auto pstage = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        As[buf][0][ap][ar0 + 16 * q] = pa[q].x;
+        if (CPLX) As[buf][P - 1][ap][ar0 + 16 * q] = pa[q].y;
+      }
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        Bs[buf][0][bp0 + BPS * q][bc] = pb[q].x;
+        if (CPLX) Bs[buf][P - 1][bp0 + BPS * q][bc] = pb[q].y;
+      }
+    };
+    int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
+    ExecTask et = exec[t];
+    setup(et, kk);
+    pfetch(min(KC, et.k - kk));
+    pstage(0);
     __syncthreads();
-    t = t2;
-    kk = kk2;
-    et = et2;
-    left = left2;
-    sp = sp2;
-    buf ^= 1;
+    int buf = 0;
+    while (left > 0) {
+      const int left2 = left - 1;
+      int t2 = t, kk2 = kk + KC;
+      if (kk2 >= et.k) {
+        t2 = t + 1;
+        kk2 = 0;
+      }
+      const bool has_next = left2 > 0;
+      ExecTask et2 = et;
+      if (has_next) {
+        if (t2 != t) {
+          et2 = exec[t2];
+          setup(et2, 0);
+        }
+        pfetch(min(KC, et2.k - kk2));
+      }
+      compute(buf);
+      if (has_next) pstage(buf ^ 1);
+      __syncthreads();
+      t = t2;
+      kk = kk2;
+      et = et2;
+      left = left2;
+      buf ^= 1;
+    }
+  } else {
+    int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
+    ExecTask et = exec[t];
+    task_rows(ga[et.a_blk], gb[et.b_blk], et);
+    int sp = span_of(et, kk, left);
+    fetch(et, ga[et.a_blk], gb[et.b_blk], kk, min(et.k, kk + KC * sp));
+    stage(0);
+    __syncthreads();
+    int buf = 0;
+    while (left > 0) {
+      const int left2 = left - sp;
+      int t2 = t, kk2 = kk + KC * sp;
+      if (kk2 >= et.k) {
+        t2 = t + 1;
+        kk2 = 0;
+      }
+      const bool has_next = left2 > 0;
+      ExecTask et2 = et;
+      int sp2 = 0;
+      if (has_next) {
+        if (t2 != t) {
+          et2 = exec[t2];
+          task_rows(ga[et2.a_blk], gb[et2.b_blk], et2);
+        }
+        sp2 = span_of(et2, kk2, left2);
+        fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2, min(et2.k, kk2 + KC * sp2));
+      }
+      compute(buf);
+      if (has_next) stage(buf ^ 1);
+      __syncthreads();
+      t = t2;
+      kk = kk2;
+      et = et2;
+      left = left2;
+      sp = sp2;
+      buf ^= 1;
+    }
   }
 
   // ---- a k-slice stores its unscaled partial sum, in the thread's own order ----
