@@ -194,32 +194,45 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     return min(U, min((e.k - kk_ + KC - 1) / KC, left_));
   };
 
-  if (seg_plain && U == 1) {
-    // Output block whose every A and B block is plain: A' element (r, p) at
-    // a_off + r*k + p, B' element (p, c) at b_off + p*n + c.  Each thread
-    // keeps one pointer per element it loads, set once per task and advanced
-    // by 16 columns (A) / 16 rows (B) per step; row and column bounds are
-    // fixed for the tile, and the k bound only matters on a task's last,
-    // partial step.
+  const bool seg_affine = (seg.flags & (kSegAColAffine | kSegBRowAffine)) == (kSegAColAffine | kSegBRowAffine);
+  if (seg_affine && U == 1) {
+    // Output block whose every task reads A and B blocks that are affine in
+    // k: A' element (r, p) at a_off + roff(r) + p * cs, B' element (p, c) at
+    // b_off + p * rs + coff(c) (plain blocks: roff = r*k, cs = 1, rs = n,
+    // coff = c).  Each thread keeps one pointer per element it loads, set
+    // once per task (the row / column offsets decomposed there) and advanced
+    // by 16 k per step; row and column bounds are fixed for the tile, and the
+    // k bound only matters on a task's last, partial step.
     const int ap = tid & 15, ar0 = tid >> 4;
     const int bc = tid % TNv, bp0 = tid / TNv;
     constexpr int BPS = 256 / TNv;                     // B' rows between a thread's elements
     bool aok[QA], bok[QB];
     const double *aptr[QA], *bptr[QB];
+    int64_t a_adv = 0, b_adv = 0;
 #pragma unroll
     for (int q = 0; q < QA; ++q) aok[q] = i0 + ar0 + 16 * q < seg.m;
 #pragma unroll
     for (int q = 0; q < QB; ++q) bok[q] = j0 + bc < seg.n;
     auto setup = [&](const ExecTask &e, int kk0) {
+      const BlkGather &gA = ga[e.a_blk];
+      const BlkGather &gB = gb[e.b_blk];
+      const bool pa_ = seg_plain || gA.plain != 0, pb_ = seg_plain || gB.plain != 0;
+      const int64_t cs = pa_ ? 1 : gA.cstep;
+      const int64_t rs = pb_ ? seg.n : gB.rstep;
+      a_adv = KC * cs * P;
+      b_adv = KC * rs * P;
 #pragma unroll
       for (int q = 0; q < QA; ++q) {
-        const int64_t r = aok[q] ? i0 + ar0 + 16 * q : 0;
-        aptr[q] = A + (e.a_off + r * e.k + ap + kk0) * P;
+        const int r = aok[q] ? i0 + ar0 + 16 * q : 0;
+        const int64_t roff = pa_ ? static_cast<int64_t>(r) * e.k
+                                 : split_offset(r, gA.rdim, gA.rstr, gA.rmag, gA.rsh, L.a_nr);
+        aptr[q] = A + (e.a_off + roff + (ap + kk0) * cs) * P;
       }
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
-        const int64_t c = bok[q] ? j0 + bc : 0;
-        bptr[q] = B + (e.b_off + static_cast<int64_t>(bp0 + BPS * q + kk0) * seg.n + c) * P;
+        const int c = bok[q] ? j0 + bc : 0;
+        const int64_t coff = pb_ ? c : split_offset(c, gB.cdim, gB.cstr, gB.cmag, gB.csh, L.b_nc);
+        bptr[q] = B + (e.b_off + static_cast<int64_t>(bp0 + BPS * q + kk0) * rs + coff) * P;
       }
     };
     auto pfetch = [&](int kvalid) {
@@ -229,14 +242,14 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
         const bool ok = aok[q] && (full || ap < kvalid);
         if (CPLX) pa[q] = ok ? *reinterpret_cast<const double2 *>(aptr[q]) : double2{0.0, 0.0};
         else pa[q] = double2{ok ? *aptr[q] : 0.0, 0.0};
-        aptr[q] += KC * P;
+        aptr[q] += a_adv;
       }
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
         const bool ok = bok[q] && (full || bp0 + BPS * q < kvalid);
         if (CPLX) pb[q] = ok ? *reinterpret_cast<const double2 *>(bptr[q]) : double2{0.0, 0.0};
         else pb[q] = double2{ok ? *bptr[q] : 0.0, 0.0};
-        bptr[q] += static_cast<int64_t>(KC) * seg.n * P;
+        bptr[q] += b_adv;
       }
     };
     auto pstage = [&](int buf) {

@@ -416,6 +416,22 @@ __global__ void k_gather_desc(const int32_t *dims, int64_t n, const PlanArgs arg
   for (int j = 0; j < nc; ++j)
     if (caxes[j] == inner) fastcol = 1;
   g.fastcol = fastcol;
+  // merge the axes of extent > 1 of each part: affine when every adjacent
+  // pair satisfies stride[j] == stride[j + 1] * dim[j + 1]
+  auto affine = [](const int32_t *dim, const int32_t *str, int n, int32_t &step) {
+    int32_t last = -1;
+    int ok = 1;
+    step = 1;
+    for (int j = 0; j < n; ++j) {
+      if (dim[j] <= 1) continue;
+      if (last >= 0 && str[last] != str[j] * dim[j]) ok = 0;
+      last = j;
+    }
+    if (last >= 0) step = str[last];
+    return ok;
+  };
+  g.caff = affine(g.cdim, g.cstr, nc, g.cstep);
+  g.raff = affine(g.rdim, g.rstr, nr, g.rstep);
   for (int q = 0; q < QLT_MAX_RANK; ++q) {
     magic_of(static_cast<uint32_t>(g.rdim[q]), g.rmag[q], g.rsh[q]);
     magic_of(static_cast<uint32_t>(g.cdim[q]), g.cmag[q], g.csh[q]);
@@ -470,11 +486,14 @@ qlt_status mark_plain_segments(qlt_plan *plan) {
   QLT_HIP(hipMemcpy(ga.data(), plan->d_ga, sizeof(BlkGather) * ga.size(), hipMemcpyDeviceToHost));
   QLT_HIP(hipMemcpy(gb.data(), plan->d_gb, sizeof(BlkGather) * gb.size(), hipMemcpyDeviceToHost));
   for (Seg &sg : plan->h_segs) {
-    int32_t f = kSegAPlain | kSegBPlain;
+    int32_t f = kSegAPlain | kSegBPlain | kSegAColAffine | kSegBRowAffine;
     for (int32_t t = sg.t0; t < sg.t1; ++t) {
       const ExecTask &e = plan->h_exec[static_cast<size_t>(t)];
-      if (!ga[static_cast<size_t>(e.a_blk)].plain) f &= ~kSegAPlain;
-      if (!gb[static_cast<size_t>(e.b_blk)].plain) f &= ~kSegBPlain;
+      const BlkGather &a = ga[static_cast<size_t>(e.a_blk)], &b = gb[static_cast<size_t>(e.b_blk)];
+      if (!a.plain) f &= ~kSegAPlain;
+      if (!b.plain) f &= ~kSegBPlain;
+      if (!a.plain && !a.caff) f &= ~kSegAColAffine;
+      if (!b.plain && !b.raff) f &= ~kSegBRowAffine;
     }
     sg.flags = f;
   }

@@ -73,6 +73,10 @@ struct BlkGather {
   int32_t ncols;   // product of cdim (the leading dimension when plain)
   int32_t nrows;   // product of rdim
   int32_t fastcol; // 1 when the block's innermost axis of extent > 1 is a col axis
+  // 1 when the col (row) axes of extent > 1 merge into one axis, so that the
+  // offset of col p (row p) is p * cstep (p * rstep)
+  int32_t caff, cstep;
+  int32_t raff, rstep;
 };
 
 // One task in execution order (sorted by output block, generation order
@@ -99,6 +103,8 @@ struct Seg {
 };
 constexpr int32_t kSegAPlain = 1;
 constexpr int32_t kSegBPlain = 2;
+constexpr int32_t kSegAColAffine = 4;   // every task's A block has affine cols (contracted axes)
+constexpr int32_t kSegBRowAffine = 8;   // every task's B block has affine rows (contracted axes)
 
 // Offset of linear index `x` (0 <= x < 2^31) inside a block, decomposed
 // row-major over dims[0..n-1] (last fastest) with the given memory strides;
