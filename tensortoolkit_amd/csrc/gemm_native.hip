@@ -155,9 +155,18 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   };
 
   // ---- MFMAs on one staged step ----
-  auto compute = [&](int buf) {
+  // Work outside the output block is skipped at the MFMA's own granularity:
+  // 16 x 16 sub-blocks of the wave that lie wholly beyond m or n, and k
+  // steps of 4 beyond the step's valid extent `kv` (all wave-uniform).
+  bool mt_ok[BM], nt_ok[BN];
+#pragma unroll
+  for (int mt = 0; mt < BM; ++mt) mt_ok[mt] = i0 + wm + mt * 16 < seg.m;
+#pragma unroll
+  for (int nt = 0; nt < BN; ++nt) nt_ok[nt] = j0 + wn + nt * 16 < seg.n;
+  auto compute = [&](int buf, int kv) {
 #pragma unroll
     for (int ks = 0; ks < KCv; ks += 4) {
+      if (ks >= kv) break;
       double a[BM], b[BN];
 #pragma unroll
       for (int mt = 0; mt < BM; ++mt) a[mt] = As[buf][0][ks + lk][wm + mt * 16 + lr];
@@ -167,7 +176,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       for (int mt = 0; mt < BM; ++mt)
 #pragma unroll
         for (int nt = 0; nt < BN; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+          if (mt_ok[mt] && nt_ok[nt])
+            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
       if (CPLX) {
         double ai[BM], bi[BN];
 #pragma unroll
@@ -179,6 +189,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
         for (int mt = 0; mt < BM; ++mt)
 #pragma unroll
           for (int nt = 0; nt < BN; ++nt) {
+            if (!(mt_ok[mt] && nt_ok[nt])) continue;
             acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bi[nt], acc_i[mt][nt], 0, 0, 0);
             acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[mt], bi[nt], acc[mt][nt], 0, 0, 0);
             acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[mt], b[nt], acc_i[mt][nt], 0, 0, 0);
@@ -267,7 +278,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
     ExecTask et = exec[t];
     setup(et, kk);
-    pfetch(min(KC, et.k - kk));
+    int kv = min(KC, et.k - kk);
+    pfetch(kv);
     pstage(0);
     __syncthreads();
     int buf = 0;
@@ -280,19 +292,22 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       }
       const bool has_next = left2 > 0;
       ExecTask et2 = et;
+      int kv2 = 0;
       if (has_next) {
         if (t2 != t) {
           et2 = exec[t2];
           setup(et2, 0);
         }
-        pfetch(min(KC, et2.k - kk2));
+        kv2 = min(KC, et2.k - kk2);
+        pfetch(kv2);
       }
-      compute(buf);
+      compute(buf, kv);
       if (has_next) pstage(buf ^ 1);
       __syncthreads();
       t = t2;
       kk = kk2;
       et = et2;
+      kv = kv2;
       left = left2;
       buf ^= 1;
     }
@@ -323,7 +338,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
         sp2 = span_of(et2, kk2, left2);
         fetch(et2, ga[et2.a_blk], gb[et2.b_blk], kk2, min(et2.k, kk2 + KC * sp2));
       }
-      compute(buf);
+      compute(buf, min(et.k - kk, KC * sp));
       if (has_next) stage(buf ^ 1);
       __syncthreads();
       t = t2;
