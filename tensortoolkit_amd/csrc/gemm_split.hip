@@ -285,6 +285,9 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int lrow = lane & 31, lk = (lane >> 5) * 8;
+  // a wave whose 32 x 32 block lies wholly outside the output block only
+  // helps to stage operands (its MFMAs and flushes are skipped)
+  const bool active = (i0 + wm < seg.m) && (j0 + wn < seg.n);
 
   double acc64[NOUT][16];
 #pragma unroll
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(256) void k_split_gemm(
       fetch(ma2, mb2, kc2);
     }
     // ---- 15 slice products (x4 real products for complex) on buffer `buf` ----
-    {
+    if (active) {
       v8bf bfr[PARTS][S];
 #pragma unroll
       for (int part = 0; part < PARTS; ++part)
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(256) void k_split_gemm(
     }
     since_flush += KC;
     // ---- flush to FP64 (always at the end of a task: exponents change) ----
-    if (since_flush >= KCH || kc == nkc - 1) {
+    if (active && (since_flush >= KCH || kc == nkc - 1)) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -481,6 +484,9 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   const int ws = w & 3, comp = w >> 2;             // spatial block, output component
   const int wm = (ws >> 1) * 32, wn = (ws & 1) * 32;
   const int lrow = lane & 31, lk = (lane >> 5) * 8;
+  // a wave whose 32 x 32 block lies wholly outside the output block only
+  // helps to stage operands (its MFMAs and flushes are skipped)
+  const bool active = (i0 + wm < seg.m) && (j0 + wn < seg.n);
 
   double acc64[16];
 #pragma unroll
@@ -546,7 +552,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
       }
       fetch(ma2, mb2, kc2);
     }
-    {
+    if (active) {
       // real part: ar*br + (-ai)*bi; imaginary part: ar*bi + ai*br
       v8bf b0[S], b1[S];
 #pragma unroll
@@ -568,7 +574,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
       }
     }
     since_flush += KC;
-    if (since_flush >= KCH || kc == nkc - 1) {
+    if (active && (since_flush >= KCH || kc == nkc - 1)) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
