@@ -16,6 +16,7 @@
 // innermost memory axis is read contiguously.
 
 #include <cstdio>
+#include <memory>
 
 #include "qlt_internal.h"
 
@@ -690,8 +691,18 @@ namespace qlt {
 // MFMA class (the split path runs it on its own kernel).
 qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                                  const double *alpha, const double *beta, hipStream_t st, bool mfma_class,
-                                 bool probe) {
+                                 bool probe, ClassStreams *shared) {
   const bool cplx = dtype == QLT_C128;
+  // classes with work run on the class streams when there are two or more
+  int n_cls = 0;
+  for (int32_t cls = mfma_class ? 0 : kNumMfmaClasses; cls < kNumTileClasses; ++cls) n_cls += plan->cls_tiles[cls] > 0;
+  std::unique_ptr<ClassStreams> own;
+  ClassStreams *cs = shared;
+  if (cs == nullptr && n_cls >= 2) {
+    own = std::make_unique<ClassStreams>(st);
+    cs = own.get();
+  }
+  auto stream_of = [&](int32_t cls) { return cs != nullptr ? cs->get(cls) : st; };
   // the class with the most algorithmic work is the one a kernel probe times
   int32_t dom = -1;
   if (probe && probe_armed(st)) {
@@ -703,16 +714,17 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
       }
   }
   auto probe_in = [&](int32_t cls) {
-    if (cls == dom) probe_begin(st);
+    if (cls == dom) probe_begin(stream_of(cls));
   };
   auto probe_out = [&](int32_t cls) {
     if (cls != dom) return;
+    const hipStream_t ps = stream_of(cls);
     static const char *const kNames[kNumTileClasses] = {"k_grouped_gemm_f64<64x64>", "k_grouped_gemm_f64<32x64>",
                                                         "k_grouped_gemm_f64<64x32>", "k_grouped_gemm_f64<32x32>",
                                                         "k_rows_f64<n<=4>", "k_rows_f64<n<=16>"};
     char name[64];
     std::snprintf(name, sizeof(name), "%s %s", kNames[cls], cplx ? "c128" : "f64");
-    probe_end(st, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], (cplx ? 16.0 : 8.0) * plan->cls_elems[cls]);
+    probe_end(ps, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], (cplx ? 16.0 : 8.0) * plan->cls_elems[cls]);
   };
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
@@ -731,13 +743,14 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
     red_base += n_red;
     if (!mfma_class || cnt == 0) continue;
     const dim3 grid(static_cast<unsigned>(cnt));
+    const hipStream_t ls = stream_of(cls);
     // k-sliced tiles: the slices write partial sums, then one reduction
     // workgroup per sliced tile adds them and applies alpha / beta
 #define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
-  hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, st, tp, plan->d_segs,       \
+  hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, ls, tp, plan->d_segs,       \
                      plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi)
 #define QLT_LAUNCH_RED(CPLXV, TMV, TNV)                                                                   \
-  hipLaunchKernelGGL((k_reduce_slices<CPLXV, TMV, TNV>), dim3(static_cast<unsigned>(n_red)), block, 0, st,  \
+  hipLaunchKernelGGL((k_reduce_slices<CPLXV, TMV, TNV>), dim3(static_cast<unsigned>(n_red)), block, 0, ls,  \
                      rp, plan->d_segs, plan->d_part, Cp, ar, ai, br, bi)
     probe_in(cls);
     if (dtype == QLT_F64) {
@@ -778,13 +791,14 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
     red_base += n_red;
     if (cnt == 0) continue;
     const dim3 grid(static_cast<unsigned>(cnt)), rgrid(static_cast<unsigned>(n_red > 0 ? n_red : 1));
+    const hipStream_t ls = stream_of(cls);
 #define QLT_LAUNCH_ROWS(CPLXV, NMAXV, RPTV)                                                                     \
   do {                                                                                                          \
-    hipLaunchKernelGGL((k_rows_f64<CPLXV, NMAXV, RPTV>), grid, block, 0, st, tp, plan->d_segs, plan->d_exec,   \
+    hipLaunchKernelGGL((k_rows_f64<CPLXV, NMAXV, RPTV>), grid, block, 0, ls, tp, plan->d_segs, plan->d_exec,   \
                        plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi);          \
     probe_out(cls);                                                                                             \
     if (n_red > 0)                                                                                              \
-      hipLaunchKernelGGL((k_reduce_rows<CPLXV, NMAXV, RPTV>), rgrid, block, 0, st, rp, plan->d_segs,           \
+      hipLaunchKernelGGL((k_reduce_rows<CPLXV, NMAXV, RPTV>), rgrid, block, 0, ls, rp, plan->d_segs,           \
                          plan->d_part, Cp, ar, ai, br, bi);                                                     \
   } while (0)
     probe_in(cls);
@@ -803,7 +817,7 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
 
 qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                          const double *alpha, const double *beta, hipStream_t st) {
-  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, true, true);
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, true, true, nullptr);
 }
 
 }  // namespace qlt

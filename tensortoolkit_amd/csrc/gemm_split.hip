@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "qlt_internal.h"
@@ -732,7 +733,7 @@ qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, con
                          const double *alpha, const double *beta, hipStream_t st);
 qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                                  const double *alpha, const double *beta, hipStream_t st, bool mfma_class,
-                                 bool probe);
+                                 bool probe, ClassStreams *shared);
 
 // Flop-weighted mean k below which a plan runs natively even in split mode:
 // such contractions are bound by memory traffic, not by the MFMA rate, and
@@ -835,6 +836,9 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
   const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
   const dim3 blk(256);
+  // the row classes (if any) go on the class streams, forked before the pack
+  std::unique_ptr<ClassStreams> rows_cs;
+  if (plan->cls_tiles[kTileRows4] > 0 || plan->cls_tiles[kTileRows16] > 0) rows_cs = std::make_unique<ClassStreams>(st);
   // a kernel probe times the split GEMM unless the row classes carry more work
   double mfma_mkn = 0.0, rows_mkn = 0.0;
   for (int32_t cls = 0; cls < kNumTileClasses; ++cls) (is_mfma_class(cls) ? mfma_mkn : rows_mkn) += plan->cls_mkn[cls];
@@ -864,8 +868,9 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
               (cplx ? 16.0 : 8.0) * elems);
   }
   QLT_HIP(hipGetLastError());
-  // narrow outputs: the native row-streaming kernel
-  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, false, !probe_split);
+  // narrow outputs: the native row-streaming kernel, on the class streams so
+  // that it overlaps the split GEMM (the output blocks are disjoint)
+  return launch_native_classes(plan, dtype, a, b, c, alpha, beta, st, false, !probe_split, rows_cs.get());
 }
 
 }  // namespace qlt

@@ -197,6 +197,29 @@ struct GatherLayout {
   int32_t b_nfast;     // B's innermost memory axis is its last saved axis
 };
 
+// Class streams: the kernels of the tile classes of one execute call write
+// disjoint output blocks, so they run on the library's class streams (default
+// priority), forked from and joined back into the caller's stream.  A
+// ClassStreams object records the fork point on construction, hands out the
+// stream of a class (making it wait for the fork on first use) and makes the
+// caller's stream wait for every used class stream on destruction.  Executes
+// that fork are serialised by a process-wide mutex held for the object's life.
+// Graph capture records the fork and join as dependencies.
+class ClassStreams {
+ public:
+  explicit ClassStreams(hipStream_t st);
+  ~ClassStreams();
+  ClassStreams(const ClassStreams &) = delete;
+  ClassStreams &operator=(const ClassStreams &) = delete;
+  hipStream_t get(int32_t cls);
+  bool ok() const { return ok_; }
+
+ private:
+  hipStream_t st_;
+  uint32_t used_ = 0;
+  bool ok_ = false;
+};
+
 // Kernel probe (qlt_kernel_probe_set): events recorded around the dominant
 // kernel of an execute call.
 bool probe_armed(hipStream_t st);

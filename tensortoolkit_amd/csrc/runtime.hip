@@ -48,6 +48,58 @@ bool stream_capturing(hipStream_t s) {
   return st == hipStreamCaptureStatusActive;
 }
 
+// ---- class streams ----
+namespace {
+std::mutex g_cls_mutex;
+hipStream_t g_cls_stream[kNumTileClasses] = {};
+hipEvent_t g_cls_fork = nullptr;
+hipEvent_t g_cls_join[kNumTileClasses] = {};
+
+hipError_t class_streams_create() {
+  for (int c = 0; c < kNumTileClasses; ++c) {
+    hipError_t e = hipStreamCreateWithFlags(&g_cls_stream[c], hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipEventCreateWithFlags(&g_cls_join[c], hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  return hipEventCreateWithFlags(&g_cls_fork, hipEventDisableTiming);
+}
+
+void class_streams_destroy() {
+  for (int c = 0; c < kNumTileClasses; ++c) {
+    if (g_cls_stream[c] != nullptr) (void)hipStreamDestroy(g_cls_stream[c]);
+    if (g_cls_join[c] != nullptr) (void)hipEventDestroy(g_cls_join[c]);
+    g_cls_stream[c] = nullptr;
+    g_cls_join[c] = nullptr;
+  }
+  if (g_cls_fork != nullptr) (void)hipEventDestroy(g_cls_fork);
+  g_cls_fork = nullptr;
+}
+}  // namespace
+
+ClassStreams::ClassStreams(hipStream_t st) : st_(st) {
+  g_cls_mutex.lock();
+  ok_ = g_cls_fork != nullptr && hipEventRecord(g_cls_fork, st) == hipSuccess;
+}
+
+hipStream_t ClassStreams::get(int32_t cls) {
+  if (!ok_) return st_;
+  if (!(used_ & (1u << cls))) {
+    if (hipStreamWaitEvent(g_cls_stream[cls], g_cls_fork, 0) != hipSuccess) return st_;
+    used_ |= 1u << cls;
+  }
+  return g_cls_stream[cls];
+}
+
+ClassStreams::~ClassStreams() {
+  for (int c = 0; c < kNumTileClasses; ++c)
+    if (used_ & (1u << c)) {
+      (void)hipEventRecord(g_cls_join[c], g_cls_stream[c]);
+      (void)hipStreamWaitEvent(st_, g_cls_join[c], 0);
+    }
+  g_cls_mutex.unlock();
+}
+
 // ---- kernel probe ----
 namespace {
 struct KernelProbe {
@@ -267,6 +319,11 @@ qlt_status qlt_init(int32_t device) {
     g_reduce_buf = nullptr;
   }
   QLT_HIP(hipMalloc(&g_reduce_buf, sizeof(double) * (2 * kReduceBlocks + 2)));
+  {
+    std::lock_guard<std::mutex> cl(g_cls_mutex);
+    class_streams_destroy();
+    QLT_HIP(class_streams_create());
+  }
   g_device = device;
   g_initialized = true;
   return QLT_OK;
@@ -278,6 +335,10 @@ qlt_status qlt_finalize(void) {
   QLT_HIP(hipDeviceSynchronize());
   if (g_reduce_buf != nullptr) QLT_HIP(hipFree(g_reduce_buf));
   g_reduce_buf = nullptr;
+  {
+    std::lock_guard<std::mutex> cl(g_cls_mutex);
+    class_streams_destroy();
+  }
   g_initialized = false;
   return QLT_OK;
 }
