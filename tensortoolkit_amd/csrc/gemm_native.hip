@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_rows_f64(
   __shared__ double Bs[P][RK][NMAX];
   __shared__ int32_t s_acol[RK], s_brow[RK], s_bcol[NMAX];
 
-  const Tile tile = tiles[blockIdx.x];
+  const Tile tile = tiles[xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const Seg seg = segs[tile.seg];
   const int tid = threadIdx.x;
   const int r0 = tile.tm * (256 * RPT);

@@ -577,7 +577,18 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
       }
       parts += ns;
     }
-    std::stable_sort(cl.begin(), cl.end(), [](const Tile &a, const Tile &b) { return a.n_chunks > b.n_chunks; });
+    if (is_mfma_class(cls)) {
+      std::stable_sort(cl.begin(), cl.end(), [](const Tile &a, const Tile &b) { return a.n_chunks > b.n_chunks; });
+    } else {
+      // row tiles are short and bound by reading A: tiles that start on the
+      // same A block rows run next to each other (and on one XCD), so the
+      // output blocks that re-read an A block find it in L2
+      auto key = [&](const Tile &t) {
+        const ExecTask &e = ex[static_cast<size_t>(t.t_begin)];
+        return std::make_pair(static_cast<int64_t>(e.a_blk), static_cast<int64_t>(t.tm));
+      };
+      std::stable_sort(cl.begin(), cl.end(), [&](const Tile &a, const Tile &b) { return key(a) < key(b); });
+    }
     out.insert(out.end(), cl.begin(), cl.end());
     plan->cls_tiles[cls] = static_cast<int64_t>(cl.size());
     plan->cls_red[cls] = n_red;
