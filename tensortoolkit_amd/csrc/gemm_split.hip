@@ -17,10 +17,10 @@
 // 2. GEMM: C' = sum over slice pairs with sa + sb <= S + 1 (15 products) of
 //    A_sa B_sb on v_mfma_f32_32x32x16_bf16.  Products of level L = sa + sb
 //    carry the weight 2^(-9(L-2)) in their BF16 values.  They go to four FP32
-//    accumulators {L=2}, {L=3}, {L=4}, {L=5,6}; each is flushed to FP64 after
-//    at most KCH k-steps, short enough that the first three stay exact
-//    (|partial sum| < 2^24 ulps of its level), and the flush sum
-//    g2 + g3 + g4 + g56 is exact in FP64.  The flushed value is scaled by
+//    accumulators {L=2}, {L=3}, {L=4}, {L=5,6}; each of the first three is
+//    flushed to FP64 before its partial sum can reach 2^24 ulps of its level
+//    (so it stays exact), {L=5,6} together with {L=4}, and the flush sum of
+//    the flushed groups is exact in FP64.  The flushed value is scaled by
 //    2^(e_row + e_col - 16) with ldexp and added to the FP64 accumulator.
 //    Truncating at level 6 leaves a relative error of order 2^-45.
 // Complex128 is the 4-real-product form Cr = Ar Br - Ai Bi, Ci = Ar Bi + Ai Br
@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "qlt_internal.h"
@@ -469,7 +470,6 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int PARTS = 2;
-  constexpr int KCH = 32;                          // k between flushes (keeps levels 2..4 exact)
   constexpr int CHUNKS = PARTS * S * 128;          // 16-byte pieces per operand per k-chunk
   constexpr int NT = 512;
   constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   const Tile tile = tiles[blockIdx.x];
   const Seg seg = segs[tile.seg];
   const int i0 = tile.tm * TM, j0 = tile.tn * TN;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ws = w & 3, comp = w >> 2;             // spatial block, output component
   const int wm = (ws >> 1) * 32, wn = (ws & 1) * 32;
   const int lrow = lane & 31, lk = (lane >> 5) * 8;
@@ -498,17 +498,29 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
 
   v4u pa[PER_THREAD], pb[PER_THREAD];
   int32_t ea_next = 0;
-  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
+  // Plane offsets of a thread's pieces within the current task's k-chunk 0
+  // (elements; a block's planes stay below 2^31 elements), recomputed when
+  // the task changes.  A chunk then is one uniform base pointer per operand
+  // plus these offsets, so the per-chunk address work is scalar.
+  uint32_t oa[PER_THREAD], ob[PER_THREAD];
+  auto task_offsets = [&](const PackMeta &ma, const PackMeta &mb) {
     const int nkc = ma.k_pad / KC;
 #pragma unroll
     for (int u = 0; u < PER_THREAD; ++u) {
-      const int c = tid + NT * u;
-      if (c < CHUNKS) {
-        const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
-        const int64_t ia = ma.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * ma.r_pad + i0 + rr) * KC + half * 8;
-        const int64_t ib = mb.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * mb.r_pad + j0 + rr) * KC + half * 8;
-        pa[u] = *reinterpret_cast<const v4u *>(planes + ia);
-        pb[u] = *reinterpret_cast<const v4u *>(planes + ib);
+      const int c = min(tid + NT * u, CHUNKS - 1);
+      const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+      oa[u] = static_cast<uint32_t>((ps * nkc * ma.r_pad + rr) * KC + half * 8);
+      ob[u] = static_cast<uint32_t>((ps * nkc * mb.r_pad + rr) * KC + half * 8);
+    }
+  };
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
+    const uint16_t *ba = planes + ma.plane_off + (static_cast<int64_t>(kc) * ma.r_pad + i0) * KC;
+    const uint16_t *bb = planes + mb.plane_off + (static_cast<int64_t>(kc) * mb.r_pad + j0) * KC;
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      if (tid + NT * u < CHUNKS) {
+        pa[u] = *reinterpret_cast<const v4u *>(ba + oa[u]);
+        pb[u] = *reinterpret_cast<const v4u *>(bb + ob[u]);
       }
     }
   };
@@ -528,11 +540,36 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
   int nkc = ma.k_pad / KC;
   int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  task_offsets(ma, mb);
   fetch(ma, mb, 0);
   if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + i0 + tid];
   stage(0);
   __syncthreads();
-  int buf = 0, since_flush = 0;
+  int buf = 0;
+
+  // ---- flushes to FP64: each FP32 group at its own exactness limit ----
+  // A level-L product is an integer multiple of 2^(-9(L-2)) of magnitude
+  // <= 2^16, and one complex k adds 2(L-1) of them to the level's group, so
+  // the group stays exact in FP32 for 256 / (2(L-1)) values of k: {L=2}
+  // for 128, {L=3} for 64, {L=4} for 42 (flushed every 32).  {L=5,6} is
+  // flushed with {L=4}.  Every group is flushed when a task ends, since the
+  // exponents change.
+  auto flush = [&](auto ng_tag) __attribute__((always_inline)) {
+    constexpr int NG = decltype(ng_tag)::value;   // groups 4-NG..3
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int e = s_ea[buf][row] + eb - 16;
+      double sum = static_cast<double>(g[2][r]) + static_cast<double>(g[3][r]);
+      if constexpr (NG >= 3) sum += static_cast<double>(g[1][r]);
+      if constexpr (NG >= 4) sum += static_cast<double>(g[0][r]);
+      acc64[r] += ldexp(sum, e);
+    }
+    g[2] = v16f{};
+    g[3] = v16f{};
+    if constexpr (NG >= 3) g[1] = v16f{};
+    if constexpr (NG >= 4) g[0] = v16f{};
+  };
 
   while (t < seg.t1) {
     int t2 = t, kc2 = kc + 1;
@@ -550,6 +587,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
         mb2 = meta_b[e2.b_blk];
         eb2 = exps[mb2.exp_off + j0 + wn + lrow];
         if (tid < TM) ea_next = exps[ma2.exp_off + i0 + tid];
+        task_offsets(ma2, mb2);
       }
       fetch(ma2, mb2, kc2);
     }
@@ -574,19 +612,18 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
         }
       }
     }
-    since_flush += KC;
-    if (active && (since_flush >= KCH || kc == nkc - 1)) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int e = s_ea[buf][row] + eb - 16;
-        const double sum = static_cast<double>(g[0][r]) + static_cast<double>(g[1][r]) +
-                           static_cast<double>(g[2][r]) + static_cast<double>(g[3][r]);
-        acc64[r] += ldexp(sum, e);
+    {
+      const int kk = (kc + 1) * KC;                // k since the task start
+      const bool last = (kc == nkc - 1);
+      if (active && (last || kk % 32 == 0)) {
+        if (last || kk % 128 == 0) {
+          flush(std::integral_constant<int, 4>{});
+        } else if (kk % 64 == 0) {
+          flush(std::integral_constant<int, 3>{});
+        } else {
+          flush(std::integral_constant<int, 2>{});
+        }
       }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) g[q] = v16f{};
-      since_flush = 0;
     }
     if (has_next) {
       stage(buf ^ 1);
@@ -750,13 +787,18 @@ bool split_profitable(const qlt_plan *plan, int32_t dtype) {
   int64_t mfma_tiles = 0;
   for (int c = 0; c < kNumMfmaClasses; ++c) mfma_tiles += plan->cls_tiles[c];
   if (mfma_tiles == 0) return false;
-  if (g_split_min_mean_k <= 0.0) return true;   // forced
   double w = 0.0, wk = 0.0, elems = 0.0;
   std::vector<char> seen_a(plan->n_a_blks, 0), seen_b(plan->n_b_blks, 0);
+  const int64_t parts = (dtype == QLT_C128) ? 2 : 1;
   for (const Seg &s : plan->h_segs) {
     if (!is_mfma_class(s.cls)) continue;
     for (int32_t t = s.t0; t < s.t1; ++t) {
       const ExecTask &e = plan->h_exec[t];
+      // the split kernels address a block's slice planes with 32-bit offsets
+      const int64_t kp = round_up(e.k, KC);
+      if (parts * S * round_up(s.m, RPAD) * kp >= (int64_t{1} << 31) ||
+          parts * S * round_up(s.n, RPAD) * kp >= (int64_t{1} << 31))
+        return false;
       const double k = e.k;
       const double f = static_cast<double>(s.m) * s.n * k;
       w += f;
@@ -771,6 +813,7 @@ bool split_profitable(const qlt_plan *plan, int32_t dtype) {
       }
     }
   }
+  if (g_split_min_mean_k <= 0.0) return true;   // forced
   if (w <= 0.0 || wk / w < g_split_min_mean_k) return false;
   const double flops = (dtype == QLT_C128 ? 8.0 : 2.0) * w;
   const double bytes = (dtype == QLT_C128 ? 16.0 : 8.0) * elems;
