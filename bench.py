@@ -7,6 +7,11 @@ step is one matvec: the four grouped contractions on cached device plans,
 inputs resident in HBM.  Throughput = algorithmic flops (sum over tasks of
 8*m*k*n, tensor_op_cost.h:384-406) / time.
 
+Before each mode is timed, the autotuner (qlt_autotune) chooses tile shapes
+per size bucket and k-slicing targets on the chain itself, outside the timed
+region, and the plans are rebuilt under its result (``--no-autotune`` times
+the built-in rules).
+
 The JSON line reports the split-BF16 path (``value``) next to the native
 FP64 path; the roofline of the dominant kernel (the library's kernel probe
 records HIP events around that one kernel launch, on its launch stream,
@@ -29,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from tensortoolkit_amd import capi, workloads as W  # noqa: E402
+from tensortoolkit_amd import autotune as TUNE  # noqa: E402
 from tensortoolkit_amd.blocksparse import ContractionPlan  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6
@@ -70,7 +76,7 @@ class MatvecRunner:
         self.esz = 16 if mv.dtype == "c128" else 8
         self.bufs = {name: capi.DeviceBuffer.from_numpy(getattr(mv, name).data) for name in ("L", "psi", "W", "R")}
         t0 = time.perf_counter()
-        self.plans, self.outs, self.out_tensors = [], [], []
+        self.plans, self.outs, self.out_tensors, self.operands_t = [], [], [], []
         x = mv.psi
         for step, axes in enumerate(W.MATVEC_AXES):
             op = (mv.L, mv.W, mv.W, mv.R)[step]
@@ -78,12 +84,28 @@ class MatvecRunner:
             plan = ContractionPlan(first, second, list(axes[0]), list(axes[1]), stream=stream)
             out = plan.output_tensor(self.dtype)
             self.plans.append(plan)
+            self.operands_t.append((first, second, list(axes[0]), list(axes[1])))
             self.out_tensors.append(out)
             self.outs.append(capi.DeviceBuffer(max(1, out.raw_size) * self.esz))
             x = out
         capi.check(capi.lib().qlt_stream_sync(stream))
         self.plan_seconds = time.perf_counter() - t0
         self.flops = [p.flops(self.dtype) for p in self.plans]
+
+    def tune_items(self):
+        """The chain as an autotuning workload (same operands and outputs)."""
+        items = []
+        for step, (first, second, a_ax, b_ax) in enumerate(self.operands_t):
+            a, b = self.operands(step)
+            items.append((first, second, a_ax, b_ax, a, b, self.outs[step].ptr))
+        return items
+
+    def rebuild_plans(self):
+        """Rebuild the plans under the library tuning now in effect."""
+        for p in self.plans:
+            p.close()
+        self.plans = [ContractionPlan(f, s, a_ax, b_ax, stream=self.stream) for f, s, a_ax, b_ax in self.operands_t]
+        capi.check(capi.lib().qlt_stream_sync(self.stream))
 
     def operands(self, step):
         names = ("L", "W", "W", "R")
@@ -333,6 +355,8 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "native", "split"],
                     help="contraction modes to time (the JSON line needs both for split_vs_native)")
     ap.add_argument("--graph-replays", type=int, default=100)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time with the built-in tile rules (default: run the autotuner per mode first, untimed)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (scales rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -377,6 +401,17 @@ def main() -> int:
     outputs = {}
     dom = int(np.argmax(runner.flops))
     for name, mode in modes:
+        tune = None
+        TUNE.set_tuning(None)
+        if not args.no_autotune:
+            t_tune = time.perf_counter()
+            res = TUNE.autotune(runner.tune_items(), runner.dtype, mode, reps=3, max_buckets=16, stream=stream)
+            tune = {"trials": res.trials, "ms_before": round(res.ms_before, 4), "ms_after": round(res.ms_after, 4),
+                    "overrides": TUNE.overrides(res.tuning), "seconds": round(time.perf_counter() - t_tune, 1)}
+            print(f"[bench] {name}: autotune {res.trials} trials in {tune['seconds']} s, chain "
+                  f"{res.ms_before:.3f} -> {res.ms_after:.3f} ms, {tune['overrides']} overrides",
+                  file=sys.stderr, flush=True)
+        runner.rebuild_plans()
         try:
             total_ms, wall, per_launch, kernel = time_mode(runner, hip, mode, args.steps, args.warmup, dom)
         except capi.QltError as e:
@@ -385,7 +420,7 @@ def main() -> int:
         outputs[name] = runner.result()
         graph_ms = time_graph(runner, hip, mode, args.graph_replays) if args.graph_replays else None
         results[name] = dict(total_ms=total_ms, wall=wall, per_launch_ms=per_launch, graph_ms=graph_ms,
-                             kernel=kernel)
+                             kernel=kernel, autotune=tune)
         print(f"[bench] {name}: {total_ms / args.steps:.3f} ms/step, per launch {np.round(per_launch, 3).tolist()}, "
               f"dominant kernel {kernel['name']} {kernel['ms']:.4f} ms"
               + (f", graph replay {graph_ms:.3f} ms/step" if graph_ms else ""), file=sys.stderr, flush=True)
@@ -451,6 +486,7 @@ def main() -> int:
         "max_rel_err_vs_oracle": {k: float(f"{v:.3e}") for k, v in errors.items()},
         "graph_replay_ms_per_step": {k: round(v["graph_ms"], 4) for k, v in results.items() if v["graph_ms"]},
         "plan_build_s": round(runner.plan_seconds, 3),
+        "autotune": {k: v["autotune"] for k, v in results.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -183,6 +183,53 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
 qlt_status qlt_set_split_min_mean_k(double min_mean_k);
 double qlt_get_split_min_mean_k(void);
 
+/* ---- tile-shape and split-k tuning; no counterpart in the reference (its
+ * per-task BLAS calls have no tiles) ----
+ * Output blocks of the MFMA tile classes (n > 16) fall into size buckets
+ * (ceil(m/32), ceil(n/32)), each capped at QLT_TUNE_BUCKETS.  A table entry
+ * tile_class[bm-1][bn-1] selects the tile shape of its bucket (0: 64x64,
+ * 1: 32x64, 2: 64x32, 3: 32x32; -1: the built-in rule, which takes 32 in a
+ * dimension when that pads at most 9/10 as much as 64).  slice_target[c] is the
+ * workgroup count that k-slicing of tile class c aims at (0: the built-in
+ * 2048; classes 4 and 5 are the narrow-output row classes).  A plan keeps
+ * the tuning in effect when it was built. */
+#define QLT_TUNE_BUCKETS 16
+#define QLT_TILE_CLASSES 6
+typedef struct {
+  int8_t tile_class[QLT_TUNE_BUCKETS][QLT_TUNE_BUCKETS];
+  int32_t slice_target[QLT_TILE_CLASSES];
+} qlt_tuning;
+qlt_status qlt_tuning_get(qlt_tuning *tuning);
+/* NULL restores the built-in rules. */
+qlt_status qlt_tuning_set(const qlt_tuning *tuning);
+
+/* One contraction of an autotuning workload: the qlt_plan_build arguments
+ * and device operands (c is overwritten). */
+typedef struct {
+  const qlt_bsdt_desc *a, *b;
+  int32_t n_ctrct;
+  const int32_t *a_ctrct_axes, *b_ctrct_axes;
+  int32_t n_a_saved;
+  const int32_t *a_saved_axes;
+  int32_t n_b_saved;
+  const int32_t *b_saved_axes;
+  const void *a_data, *b_data;
+  void *c_data;
+} qlt_tune_item;
+
+/* Autotuner: times the whole workload (every item executed in order on
+ * `stream`, best of `reps` timed runs after a warm-up) under candidate
+ * settings and installs the fastest as the library's tuning.  Coordinate
+ * descent: the size buckets present, heaviest first (by algorithmic flops),
+ * each try the four tile shapes; then each tile class tries the slice targets
+ * 512, 1024, 2048, 4096 and unsliced.  At most `max_buckets` buckets are
+ * searched (<= 0: all).  Returns the installed tuning, the workload time
+ * under the tuning in effect on entry and under the result (milliseconds),
+ * and the number of candidate settings timed. */
+qlt_status qlt_autotune(int32_t n_items, const qlt_tune_item *items, int32_t dtype, int32_t mode,
+                        int32_t reps, int32_t max_buckets, void *stream, qlt_tuning *best,
+                        double *ms_before, double *ms_after, int32_t *n_trials);
+
 /* Kernel probe, for measurement tools; no counterpart in the reference.
  * With both events non-null (hipEvent_t created on the current device),
  * every later qlt_contract_execute outside graph capture records

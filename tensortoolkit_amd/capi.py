@@ -44,6 +44,36 @@ class BsdtDesc(C.Structure):
     ]
 
 
+QLT_TUNE_BUCKETS = 16
+QLT_TILE_CLASSES = 6
+
+
+class Tuning(C.Structure):
+    """qlt_tuning: tile class per size bucket (-1: built-in rule) and the
+    k-slicing workgroup target per tile class (0: built-in, -1: unsliced)."""
+    _fields_ = [
+        ("tile_class", (C.c_int8 * QLT_TUNE_BUCKETS) * QLT_TUNE_BUCKETS),
+        ("slice_target", C.c_int32 * QLT_TILE_CLASSES),
+    ]
+
+
+class TuneItem(C.Structure):
+    _fields_ = [
+        ("a", C.POINTER(BsdtDesc)),
+        ("b", C.POINTER(BsdtDesc)),
+        ("n_ctrct", C.c_int32),
+        ("a_ctrct_axes", _i32p),
+        ("b_ctrct_axes", _i32p),
+        ("n_a_saved", C.c_int32),
+        ("a_saved_axes", _i32p),
+        ("n_b_saved", C.c_int32),
+        ("b_saved_axes", _i32p),
+        ("a_data", C.c_void_p),
+        ("b_data", C.c_void_p),
+        ("c_data", C.c_void_p),
+    ]
+
+
 class Task(C.Structure):
     _fields_ = [(name, C.c_int64) for name in (
         "a_blk_idx", "a_data_offset", "b_blk_idx", "b_data_offset",
@@ -99,6 +129,10 @@ _SIGS = {
     "qlt_contract_execute": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                               _f64p, _f64p, C.c_void_p], C.c_int32),
     "qlt_set_split_min_mean_k": ([C.c_double], C.c_int32),
+    "qlt_tuning_get": ([C.POINTER(Tuning)], C.c_int32),
+    "qlt_tuning_set": ([C.POINTER(Tuning)], C.c_int32),
+    "qlt_autotune": ([C.c_int32, C.POINTER(TuneItem), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                      C.POINTER(Tuning), C.POINTER(C.c_double), C.POINTER(C.c_double), _i32p], C.c_int32),
     "qlt_get_split_min_mean_k": ([], C.c_double),
     "qlt_kernel_probe_set": ([C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_kernel_probe_last": ([C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int32),

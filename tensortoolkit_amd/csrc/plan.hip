@@ -41,6 +41,7 @@ struct PlanArgs {
   int64_t c_stride[QLT_MAX_RANK];
   int32_t c_is_scalar;
   int32_t tile_m, tile_n;
+  int8_t cls_table[QLT_TUNE_BUCKETS][QLT_TUNE_BUCKETS];   // tile class per size bucket (qlt_tuning)
 };
 
 struct DevTask {
@@ -325,7 +326,7 @@ __global__ void k_segs(const int64_t *c_first, const int64_t *c_off, const int32
   int64_t kt = 0;
   for (int32_t t = s.t0; t < s.t1; ++t) kt += exec[t].k;
   s.k_total = kt;
-  s.cls = tile_class_of(m, n);
+  s.cls = tuned_tile_class(m, n, args.cls_table);
   s.flags = 0;      // plain flags are set on the host once the gather descriptors exist
   segs[o] = s;
   tile_cnt[o] = tiles_of(s.cls, m, n);
@@ -502,7 +503,7 @@ qlt_status mark_plain_segments(qlt_plan *plan) {
 }
 
 // k-slicing of long tiles (MFMA and row classes alike).  Per class, a step budget Q = max(kMinSliceChunks,
-// ceil(total steps / kSliceTargetWGs)) is set; a tile of w > Q steps is cut into
+// ceil(total steps / T)) is set, T = kSliceTargetWGs unless tuned (qlt_tuning); a tile of w > Q steps is cut into
 // ceil(w / Q) (at most kMaxSlices) slices of nearly equal length along its
 // (task, k-chunk) sequence, each writing an unscaled partial sum that a
 // reduction kernel adds in slice order.  Classes whose tiles are many and even
@@ -535,14 +536,17 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
                                                      : int64_t(rows_per_tile(cls)) * (cls == kTileRows4 ? 4 : 16) * 2;
     int64_t total = 0;
     for (int64_t i = base; i < base + cnt; ++i) total += seg_steps[static_cast<size_t>(h[i].seg)];
-    const int64_t q = std::max(kMinSliceChunks, (total + kSliceTargetWGs - 1) / kSliceTargetWGs);
+    // tuned workgroup target of the class (qlt_tuning; -1: unsliced)
+    const bool unsliced = plan->slice_target[cls] < 0;
+    const int64_t target = plan->slice_target[cls] > 0 ? plan->slice_target[cls] : kSliceTargetWGs;
+    const int64_t q = std::max(kMinSliceChunks, (total + target - 1) / target);
     std::vector<Tile> cl;
     int64_t parts = 0, n_red = 0;
     for (int64_t i = base; i < base + cnt; ++i) {
       Tile t = h[static_cast<size_t>(i)];
       const Seg &s = segs[static_cast<size_t>(t.seg)];
       const int64_t w = seg_steps[static_cast<size_t>(t.seg)];
-      const int64_t ns = std::min(kMaxSlices, (w + q - 1) / q);
+      const int64_t ns = unsliced ? 1 : std::min(kMaxSlices, (w + q - 1) / q);
       t.t_begin = s.t0;
       t.k_begin = 0;
       t.n_chunks = static_cast<int32_t>(w);
@@ -751,6 +755,8 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   for (int j = 0; j < n_b_saved; ++j) args.b_saved[j] = b_saved_axes[j];
   args.c_is_scalar = (c_rank == 0) ? 1 : 0;
   args.tile_m = 64;
+  const qlt_tuning tun = tuning_snapshot();
+  std::memcpy(args.cls_table, tun.tile_class, sizeof(args.cls_table));
   args.tile_n = 64;
 
   auto plan = std::make_unique<qlt_plan>();
@@ -758,6 +764,7 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
   plan->c_rank = c_rank;
   plan->c_is_scalar = args.c_is_scalar;
   plan->tile_m = args.tile_m;
+  for (int c = 0; c < kNumTileClasses; ++c) plan->slice_target[c] = tun.slice_target[c];
   plan->tile_n = args.tile_n;
   plan->n_a_blks = a->n_blks;
   plan->n_b_blks = b->n_blks;

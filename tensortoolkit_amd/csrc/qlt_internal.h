@@ -151,6 +151,19 @@ __host__ __device__ constexpr int32_t tile_class_of(int64_t m, int64_t n) {
   return n <= 4 ? kTileRows4
                 : (n <= 16 ? kTileRows16 : (prefer_32(m) ? 1 : 0) + (prefer_32(n) ? 2 : 0));
 }
+// Tile class of an output block under a tuning table (qlt_tuning): the
+// narrow-output row classes are fixed; an MFMA-class block takes the table
+// entry of its size bucket (ceil(m/32), ceil(n/32), each capped at
+// QLT_TUNE_BUCKETS) when that entry is >= 0, else the built-in rule.
+__host__ __device__ inline int32_t tuned_tile_class(int64_t m, int64_t n,
+                                                     const int8_t (&table)[QLT_TUNE_BUCKETS][QLT_TUNE_BUCKETS]) {
+  const int32_t c = tile_class_of(m, n);
+  if (!is_mfma_class(c)) return c;
+  const int64_t bm = (m + 31) / 32 < QLT_TUNE_BUCKETS ? (m + 31) / 32 : QLT_TUNE_BUCKETS;
+  const int64_t bn = (n + 31) / 32 < QLT_TUNE_BUCKETS ? (n + 31) / 32 : QLT_TUNE_BUCKETS;
+  const int8_t t = table[bm - 1][bn - 1];
+  return (t >= 0 && t < kNumMfmaClasses) ? t : c;
+}
 __host__ __device__ constexpr int32_t rows_per_tile(int32_t cls) { return cls == kTileRows4 ? 1024 : 256; }
 __host__ __device__ constexpr int64_t tiles_of(int32_t cls, int64_t m, int64_t n) {
   return is_mfma_class(cls)
@@ -222,6 +235,8 @@ class ClassStreams {
 
 // Kernel probe (qlt_kernel_probe_set): events recorded around the dominant
 // kernel of an execute call.
+// The library tuning in effect (qlt_tuning_set), read when a plan is built.
+qlt_tuning tuning_snapshot();
 bool probe_armed(hipStream_t st);
 void probe_begin(hipStream_t st);
 void probe_end(hipStream_t st, const char *name, double flops, double bytes);
@@ -264,6 +279,9 @@ struct qlt_plan {
   // tile-size class table chosen at build time
   int32_t tile_m = 64;
   int32_t tile_n = 64;
+  // k-slicing workgroup target per tile class in effect at build time
+  // (0: the built-in target, -1: unsliced)
+  int32_t slice_target[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};
   int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};  // tiles per class, class-major in d_tiles
   double cls_mkn[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};     // sum of m*k*n over the class's tasks
   double cls_elems[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};   // distinct A + B block elements + C elements
