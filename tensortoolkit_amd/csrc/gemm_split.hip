@@ -254,6 +254,21 @@ __device__ __forceinline__ v8bf neg8(v8bf x) {
   return __builtin_bit_cast(v8bf, u);
 }
 
+// Tile order for the XCDs: workgroup b runs on XCD b % 8.  Mapping b to
+// logical tile (r / G) * 8G + (b % 8) * G + r % G, with r = b / 8, hands each
+// XCD runs of G consecutive tiles of the (longest-first) tile list, i.e.
+// neighbouring 64 x 64 tiles of one output block, which read the same A row
+// panel or B column panel of slice planes and so find it in that XCD's L2;
+// the tail beyond the last full 8G group maps to itself.
+constexpr int kXcds = 8;
+constexpr int kSplitXcdRun = 8;
+__device__ __forceinline__ int split_xcd_tile(int b, int n) {
+  constexpr int span = kXcds * kSplitXcdRun;
+  if (b >= (n / span) * span) return b;
+  const int r = b / kXcds;
+  return (r / kSplitXcdRun) * span + (b % kXcds) * kSplitXcdRun + (r % kSplitXcdRun);
+}
+
 // Level group of a slice pair (sa, sb are 0-based, level = sa + sb + 2).
 __device__ __forceinline__ constexpr int group_of(int sa, int sb) {
   return (sa + sb >= 3) ? 3 : (sa + sb);
@@ -281,7 +296,7 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
   __shared__ int32_t s_ea[2][TM];
 
-  const Tile tile = tiles[blockIdx.x];
+  const Tile tile = tiles[split_xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const Seg seg = segs[tile.seg];
   const int i0 = tile.tm * TM, j0 = tile.tn * TN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -478,7 +493,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   __shared__ int32_t s_ea[2][TM];
   static_assert(sizeof(As) >= 4 * 16 * 64 * sizeof(double), "LDS exchange of the imaginary parts");
 
-  const Tile tile = tiles[blockIdx.x];
+  const Tile tile = tiles[split_xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const Seg seg = segs[tile.seg];
   const int i0 = tile.tm * TM, j0 = tile.tn * TN;
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
