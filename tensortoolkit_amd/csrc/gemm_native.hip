@@ -744,14 +744,16 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
     if (!mfma_class || cnt == 0) continue;
     const dim3 grid(static_cast<unsigned>(cnt));
     const hipStream_t ls = stream_of(cls);
+    // this class's region of the k-slice partial sums
+    double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
     // k-sliced tiles: the slices write partial sums, then one reduction
     // workgroup per sliced tile adds them and applies alpha / beta
 #define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
   hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, ls, tp, plan->d_segs,       \
-                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi)
+                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi)
 #define QLT_LAUNCH_RED(CPLXV, TMV, TNV)                                                                   \
   hipLaunchKernelGGL((k_reduce_slices<CPLXV, TMV, TNV>), dim3(static_cast<unsigned>(n_red)), block, 0, ls,  \
-                     rp, plan->d_segs, plan->d_part, Cp, ar, ai, br, bi)
+                     rp, plan->d_segs, cls_part, Cp, ar, ai, br, bi)
     probe_in(cls);
     if (dtype == QLT_F64) {
       if (cls == 0) QLT_LAUNCH_MFMA(false, 64, 64);
@@ -792,14 +794,16 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
     if (cnt == 0) continue;
     const dim3 grid(static_cast<unsigned>(cnt)), rgrid(static_cast<unsigned>(n_red > 0 ? n_red : 1));
     const hipStream_t ls = stream_of(cls);
+    // this class's region of the k-slice partial sums
+    double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
 #define QLT_LAUNCH_ROWS(CPLXV, NMAXV, RPTV)                                                                     \
   do {                                                                                                          \
     hipLaunchKernelGGL((k_rows_f64<CPLXV, NMAXV, RPTV>), grid, block, 0, ls, tp, plan->d_segs, plan->d_exec,   \
-                       plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, plan->d_part, ar, ai, br, bi);          \
+                       plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi);          \
     probe_out(cls);                                                                                             \
     if (n_red > 0)                                                                                              \
       hipLaunchKernelGGL((k_reduce_rows<CPLXV, NMAXV, RPTV>), rgrid, block, 0, ls, rp, plan->d_segs,           \
-                         plan->d_part, Cp, ar, ai, br, bi);                                                     \
+                         cls_part, Cp, ar, ai, br, bi);                                                     \
   } while (0)
     probe_in(cls);
     if (cls == kTileRows4) {

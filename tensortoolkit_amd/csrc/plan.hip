@@ -596,7 +596,9 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
     out.insert(out.end(), cl.begin(), cl.end());
     plan->cls_tiles[cls] = static_cast<int64_t>(cl.size());
     plan->cls_red[cls] = n_red;
-    part_doubles = std::max(part_doubles, parts * slice_doubles);
+    // every class gets its own region: the class kernels run concurrently
+    plan->cls_part_off[cls] = part_doubles;
+    part_doubles += parts * slice_doubles;
     base += cnt;
   }
   QLT_CHECK_ARG(static_cast<int64_t>(out.size()) < (int64_t(1) << 31), "qlt_plan_build: too many tiles after k-slicing");

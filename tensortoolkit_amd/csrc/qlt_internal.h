@@ -288,8 +288,11 @@ struct qlt_plan {
   // ---- k-slicing of long MFMA tiles (native path) ----
   qlt::ReduceItem *d_red = nullptr;     // class-major reduction items
   int64_t cls_red[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};
-  double *d_part = nullptr;             // k-slice partial sums, sized for complex128, shared by the classes
+  // k-slice partial sums, sized for complex128; each tile class owns the
+  // region starting at cls_part_off[cls] doubles (the classes run concurrently)
+  double *d_part = nullptr;
   size_t part_doubles = 0;
+  int64_t cls_part_off[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;
