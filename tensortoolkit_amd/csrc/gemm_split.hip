@@ -470,18 +470,22 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   }
 }
 
-// Complex split GEMM with the output components on separate waves: one
-// 512-thread workgroup per 64 x 64 output tile; waves 0-3 accumulate the real
-// part (ar*br - ai*bi) and waves 4-7 the imaginary part (ar*bi + ai*br) of the
-// same 2 x 2 arrangement of 32 x 32 blocks, each over all 15 slice products,
-// with the same FP32 groups and flush points as k_split_gemm (so the same
-// exactness argument holds).  Halving the accumulators per wave lets two waves
-// share each SIMD, which hides the load and barrier latency that a single
-// wave per SIMD exposes.  The imaginary halves reach the real-part waves
-// through LDS for the epilogue.
+// Complex split GEMM with the output components on separate waves, as a
+// persistent kernel: one 512-thread workgroup per CU walks the tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the 64 x 64 tile list.  Waves
+// 0-3 accumulate the real part (ar*br - ai*bi) and waves 4-7 the imaginary
+// part (ar*bi + ai*br) of the same 2 x 2 arrangement of 32 x 32 blocks, each
+// over all 15 slice products, with the FP32 level groups of k_split_gemm.
+// Two waves per SIMD hide part of the load and barrier latency.  The
+// (tile, task, k-chunk) sequence of the workgroup is one software pipeline:
+// while the MFMAs consume one chunk from LDS, the next chunk's planes are in
+// flight into registers, also across a tile boundary, so a tile's metadata
+// and first chunk load behind the previous tile's last chunk.  When a tile
+// ends, the imaginary parts reach the real-part waves through a separate LDS
+// exchange area and the real-part waves store the tile.
 __global__ __launch_bounds__(512) void k_split_gemm_c2(
-    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
-    const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const Tile *__restrict__ tiles, int32_t n_tiles, const Seg *__restrict__ segs,
+    const ExecTask *__restrict__ exec, const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
     const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int PARTS = 2;
@@ -491,18 +495,26 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   __shared__ __attribute__((aligned(16))) uint16_t As[2][PARTS][S][TM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
   __shared__ int32_t s_ea[2][TM];
-  static_assert(sizeof(As) >= 4 * 16 * 64 * sizeof(double), "LDS exchange of the imaginary parts");
+  __shared__ double xch[4][16][64];                 // imaginary parts of a finished tile
 
-  const Tile tile = tiles[split_xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
-  const Seg seg = segs[tile.seg];
-  const int i0 = tile.tm * TM, j0 = tile.tn * TN;
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ws = w & 3, comp = w >> 2;             // spatial block, output component
   const int wm = (ws >> 1) * 32, wn = (ws & 1) * 32;
   const int lrow = lane & 31, lk = (lane >> 5) * 8;
-  // a wave whose 32 x 32 block lies wholly outside the output block only
-  // helps to stage operands (its MFMAs and flushes are skipped)
-  const bool active = (i0 + wm < seg.m) && (j0 + wn < seg.n);
+  const int G = static_cast<int>(gridDim.x);
+
+  struct TileRef {
+    int i0, j0;
+    Seg seg;
+  };
+  auto tile_at = [&](int bidx) __attribute__((always_inline)) {
+    TileRef r;
+    const Tile tl = tiles[split_xcd_tile(bidx, n_tiles)];
+    r.seg = segs[tl.seg];
+    r.i0 = tl.tm * TM;
+    r.j0 = tl.tn * TN;
+    return r;
+  };
 
   double acc64[16];
 #pragma unroll
@@ -513,12 +525,12 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
 
   v4u pa[PER_THREAD], pb[PER_THREAD];
   int32_t ea_next = 0;
-  // Plane offsets of a thread's pieces within the current task's k-chunk 0
-  // (elements; a block's planes stay below 2^31 elements), recomputed when
-  // the task changes.  A chunk then is one uniform base pointer per operand
-  // plus these offsets, so the per-chunk address work is scalar.
+  // Plane offsets of a thread's pieces within a task's k-chunk 0 (elements;
+  // a block's planes stay below 2^31 elements), recomputed when the task
+  // changes.  A chunk then is one uniform base pointer per operand plus these
+  // offsets, so the per-chunk address work is scalar.
   uint32_t oa[PER_THREAD], ob[PER_THREAD];
-  auto task_offsets = [&](const PackMeta &ma, const PackMeta &mb) {
+  auto task_offsets = [&](const PackMeta &ma, const PackMeta &mb) __attribute__((always_inline)) {
     const int nkc = ma.k_pad / KC;
 #pragma unroll
     for (int u = 0; u < PER_THREAD; ++u) {
@@ -528,7 +540,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
       ob[u] = static_cast<uint32_t>((ps * nkc * mb.r_pad + rr) * KC + half * 8);
     }
   };
-  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc, int i0, int j0) __attribute__((always_inline)) {
     const uint16_t *ba = planes + ma.plane_off + (static_cast<int64_t>(kc) * ma.r_pad + i0) * KC;
     const uint16_t *bb = planes + mb.plane_off + (static_cast<int64_t>(kc) * mb.r_pad + j0) * KC;
 #pragma unroll
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
       }
     }
   };
-  auto stage = [&](int buf) {
+  auto stage = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < PER_THREAD; ++u) {
       const int c = tid + NT * u;
@@ -551,13 +563,16 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     }
   };
 
-  int t = seg.t0, kc = 0;
+  // ---- pipeline prologue: chunk 0 of the first task of the first tile ----
+  int bt = static_cast<int>(blockIdx.x);
+  TileRef cur = tile_at(bt);
+  int t = cur.seg.t0, kc = 0;
   PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
   int nkc = ma.k_pad / KC;
-  int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  int32_t eb = exps[mb.exp_off + cur.j0 + wn + lrow];
   task_offsets(ma, mb);
-  fetch(ma, mb, 0);
-  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + i0 + tid];
+  fetch(ma, mb, 0, cur.i0, cur.j0);
+  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + cur.i0 + tid];
   stage(0);
   __syncthreads();
   int buf = 0;
@@ -585,27 +600,41 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     if constexpr (NG >= 3) g[1] = v16f{};
     if constexpr (NG >= 4) g[0] = v16f{};
   };
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 
-  while (t < seg.t1) {
+  while (true) {
+    // ---- the next chunk: same task, next task, or the next tile's first ----
     int t2 = t, kc2 = kc + 1;
+    bool new_task = false;
     if (kc2 == nkc) {
       t2 = t + 1;
       kc2 = 0;
+      new_task = true;
     }
-    const bool has_next = t2 < seg.t1;
+    const bool tile_end = t2 >= cur.seg.t1;
+    const int bt2 = tile_end ? bt + G : bt;
+    const bool has_next = bt2 < n_tiles;
+    TileRef nxt = cur;
     PackMeta ma2 = ma, mb2 = mb;
     int32_t eb2 = eb;
     if (has_next) {
-      if (t2 != t) {
+      if (tile_end) {
+        nxt = tile_at(bt2);
+        t2 = nxt.seg.t0;
+      }
+      if (new_task) {
         const ExecTask e2 = exec[t2];
         ma2 = meta_a[e2.a_blk];
         mb2 = meta_b[e2.b_blk];
-        eb2 = exps[mb2.exp_off + j0 + wn + lrow];
-        if (tid < TM) ea_next = exps[ma2.exp_off + i0 + tid];
+        eb2 = exps[mb2.exp_off + nxt.j0 + wn + lrow];
+        if (tid < TM) ea_next = exps[ma2.exp_off + nxt.i0 + tid];
         task_offsets(ma2, mb2);
       }
-      fetch(ma2, mb2, kc2);
+      fetch(ma2, mb2, kc2, nxt.i0, nxt.j0);
     }
+    // a wave whose 32 x 32 block lies wholly outside the output block only
+    // helps to stage operands (its MFMAs and flushes are skipped)
+    const bool active = (cur.i0 + wm < cur.seg.m) && (cur.j0 + wn < cur.seg.n);
     if (active) {
       // real part: ar*br + (-ai)*bi; imaginary part: ar*bi + ai*br
       v8bf b0[S], b1[S];
@@ -626,11 +655,9 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
           g[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, b1[sb], g[q], 0, 0, 0);
         }
       }
-    }
-    {
       const int kk = (kc + 1) * KC;                // k since the task start
       const bool last = (kc == nkc - 1);
-      if (active && (last || kk % 32 == 0)) {
+      if (last || kk % 32 == 0) {
         if (last || kk % 128 == 0) {
           flush(std::integral_constant<int, 4>{});
         } else if (kk % 64 == 0) {
@@ -640,12 +667,47 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
         }
       }
     }
+    // ---- a finished tile: imaginary parts to the real-part waves, store ----
+    if (tile_end) {
+      if (comp == 1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xch[ws][r][lane] = acc64[r];
+      }
+      __syncthreads();
+      if (comp == 0 && active) {
+        const int col = cur.j0 + wn + lrow;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (row < cur.seg.m && col < cur.seg.n) {
+            const int64_t idx = cur.seg.c_off + static_cast<int64_t>(row) * cur.seg.n + col;
+            const double xr = acc64[r], xi = xch[ws][r][lane];
+            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+            if (use_beta) {
+              const double2 o = *cp;
+              yr += beta_r * o.x - beta_i * o.y;
+              yi += beta_r * o.y + beta_i * o.x;
+            }
+            *cp = double2{yr, yi};
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+    }
+    // ---- stage the prefetched chunk into the other buffer ----
     if (has_next) {
       stage(buf ^ 1);
-      if (tid < TM) s_ea[buf ^ 1][tid] = (t2 != t) ? ea_next : s_ea[buf][tid];
+      if (tid < TM) s_ea[buf ^ 1][tid] = new_task ? ea_next : s_ea[buf][tid];
     }
     __syncthreads();
-    if (t2 != t) {
+    if (!has_next) break;
+    if (tile_end) {
+      cur = nxt;
+      bt = bt2;
+    }
+    if (new_task) {
       ma = ma2;
       mb = mb2;
       nkc = ma.k_pad / KC;
@@ -654,33 +716,6 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     t = t2;
     kc = kc2;
     buf ^= 1;
-  }
-
-  // ---- epilogue: imaginary parts to the real-part waves through LDS ----
-  double *xch = reinterpret_cast<double *>(&As[0][0][0][0][0]);
-  if (comp == 1) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) xch[(ws * 16 + r) * 64 + lane] = acc64[r];
-  }
-  __syncthreads();
-  if (comp == 1) return;
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
-  const int col = j0 + wn + lrow;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (row < seg.m && col < seg.n) {
-      const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
-      const double xr = acc64[r], xi = xch[(ws * 16 + r) * 64 + lane];
-      double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-      double2 *cp = reinterpret_cast<double2 *>(C) + idx;
-      if (use_beta) {
-        const double2 o = *cp;
-        yr += beta_r * o.x - beta_i * o.y;
-        yi += beta_r * o.y + beta_i * o.x;
-      }
-      *cp = double2{yr, yi};
-    }
   }
 }
 
@@ -915,9 +950,11 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
     if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_gemm_c2, dim3(n_mfma), dim3(512), 0, st,
-                       stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
-                       static_cast<double *>(c), ar, ai, br, bi);
+    // persistent: one workgroup per CU (the kernel's LDS allows one)
+    const unsigned grid = std::min<unsigned>(n_mfma, static_cast<unsigned>(device_cu_count()));
+    hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st,
+                       stiles, static_cast<int32_t>(n_mfma), plan->d_segs, plan->d_exec, meta_a, meta_b, planes,
+                       exps, static_cast<double *>(c), ar, ai, br, bi);
   }
   if (probe_split) {
     double elems = 0.0;

@@ -17,6 +17,8 @@ qlt_status fail(qlt_status code, const std::string &msg);
 qlt_status hip_fail(hipError_t e, const char *what, const char *file, int line);
 bool initialized();
 int current_device();
+// Compute units of the library's device (read once).
+int device_cu_count();
 
 #define QLT_HIP(call)                                                   \
   do {                                                                  \

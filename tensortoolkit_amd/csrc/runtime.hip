@@ -42,6 +42,17 @@ qlt_status hip_fail(hipError_t e, const char *what, const char *file, int line) 
 bool initialized() { return g_initialized; }
 int current_device() { return g_device; }
 
+int device_cu_count() {
+  // read once per process for the library's device (qlt_init picks one device)
+  static std::once_flag once;
+  static int cus = 0;
+  std::call_once(once, [] {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, g_device) == hipSuccess && v > 0) cus = v;
+  });
+  return cus > 0 ? cus : 256;
+}
+
 bool stream_capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
