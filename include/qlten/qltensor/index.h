@@ -7,6 +7,8 @@
 
 #include <cassert>
 #include <functional>
+#include <ios>
+#include <iostream>
 #include <utility>
 #include <vector>
 
@@ -43,6 +45,15 @@ class QNSector {
   size_t Hash() const { return hash_; }
   bool operator==(const QNSector &rhs) const { return hash_ == rhs.hash_; }
   bool operator!=(const QNSector &rhs) const { return !(*this == rhs); }
+
+  // Stream format (qnsct.h:106-115): the quantum number, the degeneracy and
+  // the stored hash.
+  void StreamRead(std::istream &is) { is >> qn_ >> dgnc_ >> hash_; }
+  void StreamWrite(std::ostream &os) const {
+    os << qn_;
+    os << dgnc_ << "\n";
+    os << hash_ << "\n";
+  }
 
  private:
   QNT qn_;
@@ -90,6 +101,26 @@ class Index {
   bool operator==(const Index &rhs) const { return hash_ == rhs.hash_; }
   bool operator!=(const Index &rhs) const { return !(*this == rhs); }
 
+  // Stream format (index.h:170-188): sector count, the sectors, the direction
+  // as an int, the dimension and the stored hash.
+  void StreamRead(std::istream &is) {
+    size_t n = 0;
+    is >> n;
+    qnscts_ = QNSectorVec<QNT>(n);
+    for (auto &s : qnscts_) s.StreamRead(is);
+    int dir = 0;
+    is >> dir;
+    dir_ = static_cast<TenIndexDirType>(dir);
+    is >> dim_ >> hash_;
+  }
+  void StreamWrite(std::ostream &os) const {
+    os << qnscts_.size() << "\n";
+    for (const auto &s : qnscts_) s.StreamWrite(os);
+    os << static_cast<int>(dir_) << "\n";
+    os << dim_ << "\n";
+    os << hash_ << "\n";
+  }
+
  private:
   size_t CalcHash() const { return VecHasher(qnscts_) ^ std::hash<int>()(static_cast<int>(dir_)); }
   QNSectorVec<QNT> qnscts_;
@@ -97,6 +128,31 @@ class Index {
   size_t dim_ = 0;
   size_t hash_ = 0;
 };
+
+// Stream operators; a write that leaves the stream failed throws, as the
+// reference's Streamable operator<< does (framework/bases/streamable.h:58-64).
+template <typename QNT>
+std::istream &operator>>(std::istream &is, QNSector<QNT> &s) {
+  s.StreamRead(is);
+  return is;
+}
+template <typename QNT>
+std::ostream &operator<<(std::ostream &os, const QNSector<QNT> &s) {
+  s.StreamWrite(os);
+  if (!os) throw std::ios_base::failure("Failed to write streamable object.");
+  return os;
+}
+template <typename QNT>
+std::istream &operator>>(std::istream &is, Index<QNT> &ix) {
+  ix.StreamRead(is);
+  return is;
+}
+template <typename QNT>
+std::ostream &operator<<(std::ostream &os, const Index<QNT> &ix) {
+  ix.StreamWrite(os);
+  if (!os) throw std::ios_base::failure("Failed to write streamable object.");
+  return os;
+}
 
 template <typename IndexT>
 IndexT InverseIndex(const IndexT &idx) {

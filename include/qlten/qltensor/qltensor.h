@@ -14,6 +14,10 @@
 #include <cassert>
 #include <cmath>
 #include <cstdint>
+#include <ios>
+#include <istream>
+#include <ostream>
+#include <complex>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -182,6 +186,43 @@ class BlockSparseDataTensor {
       hp_numeric::SyncLibStream();
       QltCheck(qlt_memcpy_h2d(raw_.get(), v.data(), raw_.bytes()));
     }
+  }
+
+  // ---- on-disk format (BlockSparseDataTensor::StreamRead/StreamWrite,
+  // blk_spar_data_ten.h:790-830; RawDataRead_/RawDataWrite_,
+  // raw_data_operations.h:582-633) ----
+  // Text: the block count, then every block's coordinates one per line in
+  // ascending block index; then one line break and the raw data as binary
+  // ElemT (host byte order), then a line break.  A scalar with no data writes
+  // one zero element.
+  void StreamWrite(std::ostream &os) const {
+    os << blk_idx_data_blk_map_.size() << "\n";
+    for (const auto &kv : blk_idx_data_blk_map_)
+      for (size_t c : kv.second.blk_coors) os << c << "\n";
+    if (IsScalar() && GetActualRawDataSize() == 0) {
+      const ElemT zero{};
+      os.write(reinterpret_cast<const char *>(&zero), sizeof(ElemT));
+      os << "\n";
+    } else {
+      const std::vector<ElemT> h = RawDataToHost();
+      os.write(reinterpret_cast<const char *>(h.data()), static_cast<std::streamsize>(h.size() * sizeof(ElemT)));
+      os << std::endl;
+    }
+  }
+  void StreamRead(std::istream &is) {
+    size_t n = 0;
+    is >> n;
+    std::vector<CoorsT> coors(n, CoorsT(ten_rank));
+    for (auto &c : coors)
+      for (size_t j = 0; j < ten_rank; ++j) is >> c[j];
+    blk_idx_data_blk_map_.clear();
+    DataBlksInsert(coors);
+    Allocate();
+    is.seekg(1, std::ios::cur);   // the line break before the binary data
+    std::vector<ElemT> h(raw_data_size_);
+    is.read(reinterpret_cast<char *>(h.data()), static_cast<std::streamsize>(h.size() * sizeof(ElemT)));
+    if (!is) throw std::ios_base::failure("BlockSparseDataTensor::StreamRead: truncated raw data");
+    RawDataFromHost(h);
   }
 
   // Descriptor arrays handed to qlt_plan_build.
@@ -410,6 +451,55 @@ class QLTensor {
     return out;
   }
 
+  // ---- on-disk format (QLTensor::StreamRead/StreamWrite, qltensor_impl.h:798-830) ----
+  // The rank, every index, then the block-sparse data tensor.  Only a default
+  // tensor reads.
+  void StreamRead(std::istream &is) {
+    if (!IsDefault()) throw std::invalid_argument("QLTensor::StreamRead: only a default tensor can read");
+    size_t rank = 0;
+    is >> rank;
+    indexes_ = IndexVec<QNT>(rank);
+    for (auto &ix : indexes_) ix.StreamRead(is);
+    bsdt_.reset();
+    Init();
+    if (!bsdt_) throw std::ios_base::failure("QLTensor::StreamRead: the stream holds a tensor of size 0");
+    bsdt_->StreamRead(is);
+  }
+  void StreamWrite(std::ostream &os) const {
+    if (IsDefault()) throw std::invalid_argument("QLTensor::StreamWrite: default tensor");
+    os << rank_ << "\n";
+    for (const auto &ix : indexes_) {
+      ix.StreamWrite(os);
+      if (!os) throw std::ios_base::failure("Failed to write streamable object.");
+    }
+    bsdt_->StreamWrite(os);
+    if (!os) throw std::ios_base::failure("Failed to write streamable object.");
+  }
+
+  // Equality as the reference's (qltensor_impl.h:353-366, blk_spar_data_ten.h
+  // operator==): both default, or equal indexes, the same block indices and
+  // every raw element within 1e-15 (kDoubleEpsilon, framework/consts.h:19).
+  bool operator==(const QLTensor &rhs) const {
+    if (IsDefault() || rhs.IsDefault()) return IsDefault() && rhs.IsDefault();
+    if (indexes_ != rhs.indexes_) return false;
+    const auto &lm = bsdt_->GetBlkIdxDataBlkMap(), &rm = rhs.bsdt_->GetBlkIdxDataBlkMap();
+    if (!IsScalar()) {
+      if (lm.size() != rm.size()) return false;
+      for (auto l = lm.begin(), r = rm.begin(); l != lm.end(); ++l, ++r)
+        if (l->first != r->first) return false;
+    }
+    const std::vector<ElemT> a = bsdt_->RawDataToHost(), b = rhs.bsdt_->RawDataToHost();
+    if (IsScalar()) {
+      const ElemT x = a.empty() ? ElemT{} : a[0], y = b.empty() ? ElemT{} : b[0];
+      return std::abs(x - y) < 1.0e-15;
+    }
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+      if (!(std::abs(a[i] - b[i]) < 1.0e-15)) return false;
+    return true;
+  }
+  bool operator!=(const QLTensor &rhs) const { return !(*this == rhs); }
+
  private:
   void Init() {
     rank_ = indexes_.size();
@@ -506,6 +596,20 @@ class QLTensor {
   IndexVec<QNT> indexes_;
   std::unique_ptr<BSDT> bsdt_;
 };
+
+// Stream operators of a tensor (qltensor.h:702-712); writing raises
+// std::ios_base::failure when the stream fails, as the reference's block
+// data tensor and indexes do through Streamable (streamable.h:58-64).
+template <typename ElemT, typename QNT>
+std::istream &operator>>(std::istream &is, QLTensor<ElemT, QNT> &t) {
+  t.StreamRead(is);
+  return is;
+}
+template <typename ElemT, typename QNT>
+std::ostream &operator<<(std::ostream &os, const QLTensor<ElemT, QNT> &t) {
+  t.StreamWrite(os);
+  return os;
+}
 
 }  // namespace qlten
 

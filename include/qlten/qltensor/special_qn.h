@@ -35,6 +35,10 @@ class U1QN {
   int GetVal() const { return val_; }
   static U1QN Zero() { return U1QN(0); }
 
+  // Stream format (u1qn.h:126-133): the value and the stored hash, one per line.
+  void StreamRead(std::istream &is) { is >> val_ >> hash_; }
+  void StreamWrite(std::ostream &os) const { os << val_ << "\n" << hash_ << "\n"; }
+
  private:
   // the value, rotated left by 31 bits (u1qn.h:139-152)
   static size_t CalcHash(int v) {
@@ -68,6 +72,10 @@ class U1U1QN {
   int GetVal(size_t i) const { return vals_[i]; }
   static U1U1QN Zero() { return U1U1QN(0, 0); }
 
+  // Stream format (u1u1qn.h:131-139): both values and the stored hash, one per line.
+  void StreamRead(std::istream &is) { is >> vals_[0] >> vals_[1] >> hash_; }
+  void StreamWrite(std::ostream &os) const { os << vals_[0] << "\n" << vals_[1] << "\n" << hash_ << "\n"; }
+
  private:
   // both values offset by 2^30 and packed as v1 + 2^31 v2, rotated left by
   // 10 bits (u1u1qn.h:151-173)
@@ -79,6 +87,23 @@ class U1U1QN {
   int vals_[2];
   size_t hash_;
 };
+
+inline std::istream &operator>>(std::istream &is, U1QN &qn) {
+  qn.StreamRead(is);
+  return is;
+}
+inline std::ostream &operator<<(std::ostream &os, const U1QN &qn) {
+  qn.StreamWrite(os);
+  return os;
+}
+inline std::istream &operator>>(std::istream &is, U1U1QN &qn) {
+  qn.StreamRead(is);
+  return is;
+}
+inline std::ostream &operator<<(std::ostream &os, const U1U1QN &qn) {
+  qn.StreamWrite(os);
+  return os;
+}
 
 }  // namespace special_qn
 

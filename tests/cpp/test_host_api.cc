@@ -9,7 +9,11 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
 #include <functional>
+#include <sstream>
+#include <streambuf>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -163,6 +167,54 @@ void CheckContract(const QLTensor<T, U1QN> &a, const QLTensor<T, U1QN> &b, const
   const double err = RelDiff(got, ref);
   CHECK(err < 1e-13, name + ": rel err " + std::to_string(err));
   CHECK(c.Rank() == a_sv.size() + b_sv.size(), name + ": rank");
+}
+
+// .qlten round trip through a file (RunTestQLTensorFileIOCase,
+// tests/test_qltensor/test_qltensor.cc:1029-1041).
+template <typename TenT>
+void CheckFileIO(const TenT &t, const std::string &name) {
+  const std::string file = "test_host_api_io.qlten";
+  {
+    std::ofstream out(file, std::ofstream::binary);
+    out << t;
+  }
+  TenT cpy;
+  {
+    std::ifstream in(file, std::ifstream::binary);
+    in >> cpy;
+  }
+  std::remove(file.c_str());
+  CHECK(cpy == t, "file IO round trip: " + name);
+}
+
+// A stream buffer that accepts `limit` bytes and then fails
+// (test_qltensor.cc:37-65).
+class WriteFailAfter : public std::streambuf {
+ public:
+  explicit WriteFailAfter(std::streamsize limit) : left_(limit) {}
+
+ protected:
+  std::streamsize xsputn(const char *, std::streamsize n) override {
+    if (left_ <= 0) return 0;
+    const std::streamsize w = std::min(n, left_);
+    left_ -= w;
+    return w;
+  }
+  int_type overflow(int_type ch) override {
+    if (traits_type::eq_int_type(ch, traits_type::eof())) return traits_type::not_eof(ch);
+    if (left_ <= 0) return traits_type::eof();
+    --left_;
+    return traits_type::not_eof(ch);
+  }
+
+ private:
+  std::streamsize left_;
+};
+
+DTen d2_for_fail(const Fixture &f) {
+  DTen d2({f.idx_in_s, f.idx_out_s});
+  d2.Random(U1QN(0));
+  return d2;
 }
 
 int main() {
@@ -407,6 +459,74 @@ int main() {
     for (size_t i = 0; i < k; ++i)
       for (size_t j = 0; j < k; ++j) dev = std::max(dev, std::abs(dq[i * k + j] - QLTEN_Complex(i == j ? 1.0 : 0.0)));
     CHECK(dev < 1e-12, "qr: Q^dag Q == 1");
+  }
+
+  // ---- .qlten file format (QLTensor::StreamRead/StreamWrite, qltensor_impl.h:798-830;
+  // the cases of tests/test_qltensor/test_qltensor.cc:1043-1073) ----
+  {
+    const U1QN qn0(0), qnp1(1);
+    DTen dscalar{IndexVec<U1QN>{}};
+    dscalar.Random(U1QN());
+    CheckFileIO(dscalar, "real scalar");
+    ZTen zscalar{IndexVec<U1QN>{}};
+    zscalar.Random(U1QN());
+    CheckFileIO(zscalar, "complex scalar");
+    for (const U1QN &q : {qn0, qnp1}) {
+      DTen d1({f.idx_out_s});
+      d1.Random(q);
+      CheckFileIO(d1, "real 1d");
+      DTen d2({f.idx_in_s, f.idx_out_s});
+      d2.Random(q);
+      CheckFileIO(d2, "real 2d");
+      DTen d3({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+      d3.Random(q);
+      CheckFileIO(d3, "real 3d");
+      ZTen z3({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+      z3.Random(q);
+      CheckFileIO(z3, "complex 3d");
+    }
+    using U11Ten = QLTensor<QLTEN_Complex, U1U1QN>;
+    const Index<U1U1QN> u_in({QNSector<U1U1QN>(U1U1QN(0, 0), 2), QNSector<U1U1QN>(U1U1QN(1, -1), 3),
+                              QNSector<U1U1QN>(U1U1QN(-1, 1), 1)},
+                             TenIndexDirType::IN);
+    const Index<U1U1QN> u_out = InverseIndex(u_in);
+    U11Ten u3({u_in, u_out, u_out});
+    u3.Random(U1U1QN(0, 0));
+    CheckFileIO(u3, "complex U1U1 3d");
+
+    // the read tensor is an ordinary tensor: it contracts like the original
+    DTen a2({f.idx_in_s, f.idx_out_l}), b2({f.idx_in_l, f.idx_out_s});
+    a2.Random(qn0);
+    b2.Random(qnp1);
+    std::stringstream ss;
+    ss << a2;
+    DTen a2r;
+    ss >> a2r;
+    DTen c0, c1;
+    Contract(&a2, &b2, {{1}, {0}}, &c0);
+    Contract(&a2r, &b2, {{1}, {0}}, &c1);
+    CHECK(c0 == c1, "a read tensor contracts as the original");
+
+    // a write into a failing stream throws (test_qltensor.cc:1075-1082)
+    WriteFailAfter failing(32);
+    std::ostream bad(&failing);
+    bool threw = false;
+    try {
+      bad << d2_for_fail(f);
+    } catch (const std::ios_base::failure &) {
+      threw = true;
+    }
+    CHECK(threw, "writing into a failing stream throws std::ios_base::failure");
+
+    // a file for the Python reader / writer cross-check
+    if (const char *path = std::getenv("QLTEN_IO_OUT")) {
+      std::ofstream out(path, std::ofstream::binary);
+      out << u3;
+      std::ofstream out2(std::string(path) + ".real", std::ofstream::binary);
+      DTen d3({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+      d3.Random(qnp1);
+      out2 << d3;
+    }
   }
 
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
