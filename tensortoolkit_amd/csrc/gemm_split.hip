@@ -274,26 +274,25 @@ __device__ __forceinline__ constexpr int group_of(int sa, int sb) {
   return (sa + sb >= 3) ? 3 : (sa + sb);
 }
 
-// Split GEMM: one 256-thread workgroup per 64 x 64 output tile, 2 x 2 waves of
-// 32 x 32, one 32x32x16 MFMA block per slice product.  The (task, k-chunk)
-// sequence of the tile's output block is walked as one software pipeline:
-// while the MFMAs consume the chunk staged in one LDS buffer, the next
-// chunk's planes (possibly of the next task) are already in flight into
-// registers, and are written to the other LDS buffer afterwards, so there
-// is a single barrier per chunk and the load latency hides behind MFMA work.
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_split_gemm(
+// Real split GEMM: one 256-thread workgroup per 64 x 64 output tile, 2 x 2
+// waves of 32 x 32, one 32x32x16 MFMA block per slice product.  The (task,
+// k-chunk) sequence of the tile's output block is walked as one software
+// pipeline: while the MFMAs consume the chunk staged in one LDS buffer, the
+// next chunk's planes (possibly of the next task) are already in flight into
+// registers, and are written to the other LDS buffer afterwards, so there is
+// a single barrier per chunk and the load latency hides behind MFMA work.
+// The FP32 level groups are flushed to FP64 every KCH = 64 k (levels 2..4
+// stay exact) and at the end of every task (the exponents change).
+__global__ __launch_bounds__(256) void k_split_gemm_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
     const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
-    double alpha_r, double alpha_i, double beta_r, double beta_i) {
-  constexpr int PARTS = CPLX ? 2 : 1;
-  constexpr int KCH = CPLX ? 32 : 64;   // k between flushes (keeps levels 2..4 exact)
-  constexpr int NOUT = CPLX ? 2 : 1;
-  constexpr int CHUNKS = PARTS * S * 128;          // 16-byte pieces per operand per k-chunk
+    double alpha, double beta) {
+  constexpr int KCH = 64;
+  constexpr int CHUNKS = S * 128;                  // 16-byte pieces per operand per k-chunk
   constexpr int PER_THREAD = (CHUNKS + 255) / 256;
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][PARTS][S][TM][LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][S][TM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][TN][LDK];
   __shared__ int32_t s_ea[2][TM];
 
   const Tile tile = tiles[split_xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
@@ -306,16 +305,12 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   // helps to stage operands (its MFMAs and flushes are skipped)
   const bool active = (i0 + wm < seg.m) && (j0 + wn < seg.n);
 
-  double acc64[NOUT][16];
+  double acc64[16];
 #pragma unroll
-  for (int o = 0; o < NOUT; ++o)
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+  v16f g[4];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc64[o][r] = 0.0;
-  v16f g[NOUT][4];
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
+  for (int q = 0; q < 4; ++q) g[q] = v16f{};
 
   v4u pa[PER_THREAD], pb[PER_THREAD];
   int32_t ea_next = 0;
@@ -340,8 +335,8 @@ __global__ __launch_bounds__(256) void k_split_gemm(
       const int c = tid + 256 * u;
       if (c < CHUNKS) {
         const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
-        *reinterpret_cast<v4u *>(&As[buf][ps / S][ps % S][rr][half * 8]) = pa[u];
-        *reinterpret_cast<v4u *>(&Bs[buf][ps / S][ps % S][rr][half * 8]) = pb[u];
+        *reinterpret_cast<v4u *>(&As[buf][ps][rr][half * 8]) = pa[u];
+        *reinterpret_cast<v4u *>(&Bs[buf][ps][rr][half * 8]) = pb[u];
       }
     }
   };
@@ -377,32 +372,18 @@ __global__ __launch_bounds__(256) void k_split_gemm(
       }
       fetch(ma2, mb2, kc2);
     }
-    // ---- 15 slice products (x4 real products for complex) on buffer `buf` ----
+    // ---- 15 slice products on buffer `buf` ----
     if (active) {
-      v8bf bfr[PARTS][S];
+      v8bf bfr[S];
 #pragma unroll
-      for (int part = 0; part < PARTS; ++part)
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2)
-          bfr[part][s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][part][s2][wn + lrow][lk]);
+      for (int s2 = 0; s2 < S; ++s2) bfr[s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][s2][wn + lrow][lk]);
 #pragma unroll
       for (int sa = 0; sa < S; ++sa) {
-        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][0][sa][wm + lrow][lk]);
-        v8bf ai = ar, nai = ar;
-        if (CPLX) {
-          ai = *reinterpret_cast<const v8bf *>(&As[buf][PARTS - 1][sa][wm + lrow][lk]);
-          nai = neg8(ai);
-        }
+        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][sa][wm + lrow][lk]);
 #pragma unroll
         for (int sb = 0; sb < S - sa; ++sb) {
           const int q = group_of(sa, sb);
-          g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[0][sb], g[0][q], 0, 0, 0);
-          if (CPLX) {
-            g[NOUT - 1][q] =
-                __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[PARTS - 1][sb], g[NOUT - 1][q], 0, 0, 0);
-            g[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nai, bfr[PARTS - 1][sb], g[0][q], 0, 0, 0);
-            g[NOUT - 1][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, bfr[0][sb], g[NOUT - 1][q], 0, 0, 0);
-          }
+          g[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar, bfr[sb], g[q], 0, 0, 0);
         }
       }
     }
@@ -413,17 +394,12 @@ __global__ __launch_bounds__(256) void k_split_gemm(
       for (int r = 0; r < 16; ++r) {
         const int row = wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int e = s_ea[buf][row] + eb - 16;
-#pragma unroll
-        for (int o = 0; o < NOUT; ++o) {
-          const double sum = static_cast<double>(g[o][0][r]) + static_cast<double>(g[o][1][r]) +
-                             static_cast<double>(g[o][2][r]) + static_cast<double>(g[o][3][r]);
-          acc64[o][r] += ldexp(sum, e);
-        }
+        const double sum = static_cast<double>(g[0][r]) + static_cast<double>(g[1][r]) +
+                           static_cast<double>(g[2][r]) + static_cast<double>(g[3][r]);
+        acc64[r] += ldexp(sum, e);
       }
 #pragma unroll
-      for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) g[o][q] = v16f{};
+      for (int q = 0; q < 4; ++q) g[q] = v16f{};
       since_flush = 0;
     }
     // ---- stage the prefetched chunk into the other buffer ----
@@ -444,28 +420,15 @@ __global__ __launch_bounds__(256) void k_split_gemm(
   }
 
   // ---- epilogue ----
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
   const int col = j0 + wn + lrow;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (row < seg.m && col < seg.n) {
       const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
-      if (CPLX) {
-        const double xr = acc64[0][r], xi = acc64[NOUT - 1][r];
-        double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-        double2 *cp = reinterpret_cast<double2 *>(C) + idx;
-        if (use_beta) {
-          const double2 o = *cp;
-          yr += beta_r * o.x - beta_i * o.y;
-          yi += beta_r * o.y + beta_i * o.x;
-        }
-        *cp = double2{yr, yi};
-      } else {
-        double y = alpha_r * acc64[0][r];
-        if (use_beta) y += beta_r * C[idx];
-        C[idx] = y;
-      }
+      double y = alpha * acc64[r];
+      if (beta != 0.0) y += beta * C[idx];
+      C[idx] = y;
     }
   }
 }
@@ -475,7 +438,7 @@ __global__ __launch_bounds__(256) void k_split_gemm(
 // blockIdx.x, blockIdx.x + gridDim.x, ... of the 64 x 64 tile list.  Waves
 // 0-3 accumulate the real part (ar*br - ai*bi) and waves 4-7 the imaginary
 // part (ar*bi + ai*br) of the same 2 x 2 arrangement of 32 x 32 blocks, each
-// over all 15 slice products, with the FP32 level groups of k_split_gemm.
+// over all 15 slice products, with the FP32 level groups of k_split_gemm_f64.
 // Two waves per SIMD hide part of the load and barrier latency.  The
 // (tile, task, k-chunk) sequence of the workgroup is one software pipeline:
 // while the MFMAs consume one chunk from LDS, the next chunk's planes are in
@@ -942,9 +905,9 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
                        static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
     if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_gemm<false>, dim3(n_mfma), blk, 0, st,
+    hipLaunchKernelGGL(k_split_gemm_f64, dim3(n_mfma), blk, 0, st,
                        stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
-                       static_cast<double *>(c), ar, ai, br, bi);
+                       static_cast<double *>(c), ar, br);
   } else {
     hipLaunchKernelGGL(k_pack<true>, dim3(n_items), blk, 0, st, items, meta_a,
                        meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
@@ -959,7 +922,7 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   if (probe_split) {
     double elems = 0.0;
     for (int32_t cls = 0; cls < kNumMfmaClasses; ++cls) elems += plan->cls_elems[cls];
-    probe_end(st, cplx ? "k_split_gemm_c2 c128" : "k_split_gemm f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
+    probe_end(st, cplx ? "k_split_gemm_c2 c128" : "k_split_gemm_f64 f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
               (cplx ? 16.0 : 8.0) * elems);
   }
   QLT_HIP(hipGetLastError());
