@@ -40,6 +40,7 @@ from tensortoolkit_amd.blocksparse import ContractionPlan  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_INT8_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -185,6 +186,9 @@ def time_graph(runner: MatvecRunner, hip: Hip, mode: int, replays: int):
 # Slice products per algorithmic product in the split-BF16 GEMM: five BF16
 # slices per operand, every slice pair (i, j) with i + j <= 4.
 SPLIT_PRODUCTS = 15
+# Digit products per algorithmic product in the int8 split GEMM: six
+# base-255 digits per operand, every digit pair (i, j) with i + j <= 5.
+I8_PRODUCTS = 21
 
 
 def profile_traffic(kernel_name: str, workload: str):
@@ -218,6 +222,7 @@ def kernel_roofline(kernel: dict, workload: str) -> dict:
     secs = kernel["ms"] * 1e-3
     fp64_tf = kernel["flops"] / secs / 1e12
     split = kernel["name"].startswith("k_split_gemm")
+    split_i8 = kernel["name"].startswith("k_split_i8")
     rows = kernel["name"].startswith("k_rows")
     traffic, src = profile_traffic(kernel["name"], workload)
     out = {"kernel": kernel["name"], "ms_per_launch": round(kernel["ms"], 5),
@@ -226,6 +231,11 @@ def kernel_roofline(kernel: dict, workload: str) -> dict:
         gbs = kernel["bytes"] / secs / 1e9
         out.update(bound="hbm", achieved=round(gbs, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                    frac=round(gbs / PEAK_HBM_GBS, 4))
+    elif split_i8:
+        i8_tops = I8_PRODUCTS * fp64_tf
+        out.update(bound="mfma", achieved=round(i8_tops, 2), peak=PEAK_INT8_TOPS, unit="TOP/s",
+                   frac=round(i8_tops / PEAK_INT8_TOPS, 4), fp64_equivalent_tflops=round(fp64_tf, 3),
+                   fp64_equivalent_frac_of_fp64_peak=round(fp64_tf / PEAK_FP64_TFLOPS, 4))
     elif split:
         bf16_tf = SPLIT_PRODUCTS * fp64_tf
         out.update(bound="mfma", achieved=round(bf16_tf, 2), peak=PEAK_BF16_TFLOPS, unit="TFLOP/s",
@@ -305,7 +315,8 @@ def run_e2e(args) -> int:
     print(f"[e2e] tensors built in {time.perf_counter() - t0:.1f}s, psi {mv.psi.raw_size} elements",
           file=sys.stderr, flush=True)
     steps, cutoff, dmax = args.lanczos_steps, 1e-10, 4096
-    mode = capi.QLT_MODE_SPLIT_BF16 if args.e2e_mode == "split" else capi.QLT_MODE_NATIVE
+    mode = {"split": capi.QLT_MODE_SPLIT_BF16, "split_i8": capi.QLT_MODE_SPLIT_I8,
+            "native": capi.QLT_MODE_NATIVE}[args.e2e_mode]
     hv = LZ.DeviceMatvec(mv, mode, stream)
     warm_in = capi.DeviceBuffer.from_numpy(mv.psi.data)
     warm_out = capi.DeviceBuffer(mv.psi.raw_size * 16)
@@ -353,8 +364,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="both", choices=["both", "native", "split"],
-                    help="contraction modes to time (the JSON line needs both for split_vs_native)")
+    ap.add_argument("--mode", default="all", choices=["all", "both", "native", "split", "split_bf16", "split_i8"],
+                    help="contraction modes to time: all = native, split_bf16 and split_i8; both = native and "
+                         "split_bf16; split = both split modes (the JSON line needs native for split_vs_native)")
     ap.add_argument("--graph-replays", type=int, default=100)
     ap.add_argument("--autotune", action="store_true",
                     help="run the autotuner on the chain per mode first (untimed) and time under its result; "
@@ -364,7 +376,7 @@ def main() -> int:
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
     ap.add_argument("--e2e", action="store_true",
                     help="run the end-to-end two-site update (Lanczos + block SVD) instead of the Contract benchmark")
-    ap.add_argument("--e2e-mode", default="split", choices=["split", "native"])
+    ap.add_argument("--e2e-mode", default="split", choices=["split", "split_i8", "native"])
     ap.add_argument("--lanczos-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -397,8 +409,11 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     results = {}
-    modes = [("native", capi.QLT_MODE_NATIVE), ("split_bf16", capi.QLT_MODE_SPLIT_BF16)]
-    if args.mode != "both":
+    modes = [("native", capi.QLT_MODE_NATIVE), ("split_bf16", capi.QLT_MODE_SPLIT_BF16),
+             ("split_i8", capi.QLT_MODE_SPLIT_I8)]
+    if args.mode == "both":
+        modes = modes[:2]
+    elif args.mode != "all":
         modes = [m for m in modes if m[0].startswith(args.mode)]
     outputs = {}
     dom = int(np.argmax(runner.flops))
@@ -444,7 +459,10 @@ def main() -> int:
                     worst = max(worst, float(np.linalg.norm(got[o:o + sz] - ref[o:o + sz]) / den))
             errors[name] = worst
 
-    primary = "split_bf16" if "split_bf16" in results else "native"
+    primary = next((m for m in ("split_bf16", "split_i8", "native") if m in results), None)
+    if primary is None:
+        print("[bench] no mode ran", file=sys.stderr, flush=True)
+        return 1
     r = results[primary]
     ms_step = r["total_ms"] / args.steps
     value = flops_step / (ms_step * 1e-3) / 1e9
@@ -463,6 +481,17 @@ def main() -> int:
     native_gf = None
     if "native" in results:
         native_gf = flops_step / (results["native"]["total_ms"] / args.steps * 1e-3) / 1e9
+    def split_i8_summary():
+        """The int8-digit split mode beside the metric's split-BF16 one."""
+        if "split_i8" not in results:
+            return None
+        res = results["split_i8"]
+        gf = flops_step / (res["total_ms"] / args.steps * 1e-3) / 1e9
+        return {"gflops": round(gf, 2), "ms_per_step": round(res["total_ms"] / args.steps, 4),
+                "vs_native": round(gf / native_gf, 3) if native_gf else None,
+                "roofline": kernel_roofline(res["kernel"], mv.name),
+                "graph_replay_ms_per_step": round(res["graph_ms"], 4) if res["graph_ms"] else None}
+
     line = {
         "metric": "block-sparse Contract GFLOP/s, split-BF16 grouped vs FP64, at <=1e-12 rel err",
         "value": round(value, 2),
@@ -489,6 +518,7 @@ def main() -> int:
         "graph_replay_ms_per_step": {k: round(v["graph_ms"], 4) for k, v in results.items() if v["graph_ms"]},
         "plan_build_s": round(runner.plan_seconds, 3),
         "autotune": {k: v["autotune"] for k, v in results.items()},
+        "split_i8": split_i8_summary(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

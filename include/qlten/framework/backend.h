@@ -54,9 +54,9 @@ inline void *LibStream() {
 
 inline void SyncLibStream() { QltCheck(qlt_stream_sync(LibStream())); }
 
-// Arithmetic of Contract: native FP64 MFMA (default, the drop-in) or the
-// split-BF16 emulation of FP64.
-enum class ContractPrecision { kNativeFP64, kSplitBF16 };
+// Arithmetic of Contract: native FP64 MFMA (default, the drop-in), or FP64
+// emulated by BF16 slices or by int8 digits (both within 1e-12 of FP64).
+enum class ContractPrecision { kNativeFP64, kSplitBF16, kSplitInt8 };
 
 inline ContractPrecision &ContractPrecisionSetting() {
   static ContractPrecision p = ContractPrecision::kNativeFP64;
@@ -66,7 +66,14 @@ inline ContractPrecision &ContractPrecisionSetting() {
 inline void SetContractPrecision(ContractPrecision p) { ContractPrecisionSetting() = p; }
 
 inline int32_t ContractModeFlag() {
-  return ContractPrecisionSetting() == ContractPrecision::kSplitBF16 ? QLT_MODE_SPLIT_BF16 : QLT_MODE_NATIVE;
+  switch (ContractPrecisionSetting()) {
+    case ContractPrecision::kSplitBF16:
+      return QLT_MODE_SPLIT_BF16;
+    case ContractPrecision::kSplitInt8:
+      return QLT_MODE_SPLIT_I8;
+    default:
+      return QLT_MODE_NATIVE;
+  }
 }
 
 }  // namespace hp_numeric

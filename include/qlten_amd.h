@@ -55,8 +55,14 @@ enum { QLT_F64 = 0, QLT_C128 = 1 };
  *   QLT_MODE_NATIVE      FP64 MFMA (complex128 as four real FP64 MFMA products).
  *   QLT_MODE_SPLIT_BF16  FP64 emulated by exactly scaled multi-term BF16 slices on
  *                        the BF16 MFMA units, FP32 accumulation over k-chunks short
- *                        enough to stay exact, FP64 recombination. */
-enum { QLT_MODE_NATIVE = 0, QLT_MODE_SPLIT_BF16 = 1 };
+ *                        enough to stay exact, FP64 recombination.
+ *   QLT_MODE_SPLIT_I8    FP64 emulated by exactly scaled int8 digit slices (six
+ *                        base-255 digits per element) on the int8 MFMA units, which
+ *                        run at twice the BF16 rate; exact int32 accumulation per
+ *                        digit level, FP64 recombination.
+ * Both split modes keep every output block within 1e-12 relative error of FP64;
+ * neither has a counterpart in the reference, whose Contract computes in FP64. */
+enum { QLT_MODE_NATIVE = 0, QLT_MODE_SPLIT_BF16 = 1, QLT_MODE_SPLIT_I8 = 2 };
 
 #define QLT_MAX_RANK 8
 
@@ -175,7 +181,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
 
 /* Split-mode dispatch threshold: a plan whose flop-weighted mean contracted
  * extent k is below `min_mean_k`, or whose flops per byte of distinct operand
- * blocks are below 96, runs on the native kernel even in QLT_MODE_SPLIT_BF16
+ * blocks are below 96, runs on the native kernel even in a split mode
  * (such contractions are bound by memory traffic, where slicing only adds
  * traffic).  Default 48; 0 forces the split kernel.  A
  * runtime knob in the spirit of the reference's SetCpuNumericsThreads

@@ -17,7 +17,7 @@ LIB_PATH = _PKG / "libqlten_amd.so"
 
 QLT_OK = 0
 QLT_F64, QLT_C128 = 0, 1
-QLT_MODE_NATIVE, QLT_MODE_SPLIT_BF16 = 0, 1
+QLT_MODE_NATIVE, QLT_MODE_SPLIT_BF16, QLT_MODE_SPLIT_I8 = 0, 1, 2
 QLT_MAX_RANK = 8
 
 _i64p = C.POINTER(C.c_int64)

@@ -149,7 +149,7 @@ qlt_status qlt_autotune(int32_t n_items, const qlt_tune_item *items, int32_t dty
   QLT_CHECK_ARG(n_items > 0 && items != nullptr, "qlt_autotune: empty workload");
   QLT_CHECK_ARG(reps >= 1, "qlt_autotune: reps must be >= 1");
   QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_autotune: dtype must be QLT_F64 or QLT_C128");
-  QLT_CHECK_ARG(mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16, "qlt_autotune: unknown mode");
+  QLT_CHECK_ARG(valid_mode(mode), "qlt_autotune: unknown mode");
   hipStream_t st = static_cast<hipStream_t>(stream);
   QLT_CHECK_ARG(!stream_capturing(st), "qlt_autotune: cannot tune under graph capture");
   const qlt_tuning entry = tuning_snapshot();

@@ -8,11 +8,11 @@
 namespace qlt {
 qlt_status launch_native(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                          const double *alpha, const double *beta, hipStream_t st);
-qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *bytes);
-qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, int32_t mode, size_t *bytes);
+qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const void *a, const void *b, void *c,
                         const double *alpha, const double *beta, hipStream_t st);
-qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype);
-bool split_profitable(const qlt_plan *plan, int32_t dtype);
+qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype, int32_t mode);
+bool split_profitable(const qlt_plan *plan, int32_t dtype, int32_t mode);
 }  // namespace qlt
 
 using namespace qlt;
@@ -23,20 +23,21 @@ qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode) {
   QLT_REQUIRE_INIT();
   QLT_CHECK_ARG(plan != nullptr, "qlt_plan_prepare: null plan");
   QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_plan_prepare: bad dtype");
-  QLT_CHECK_ARG(mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16, "qlt_plan_prepare: bad mode");
+  QLT_CHECK_ARG(valid_mode(mode), "qlt_plan_prepare: bad mode");
   if (mode == QLT_MODE_NATIVE || plan->n_tiles == 0) return QLT_OK;
-  if (!split_profitable(plan, dtype)) return QLT_OK;
+  if (!split_profitable(plan, dtype, mode)) return QLT_OK;
   size_t need = 0;
-  QLT_TRY(split_workspace_bytes(plan, dtype, &need));
+  QLT_TRY(split_workspace_bytes(plan, dtype, mode, &need));
   if (need > plan->work_bytes) {
     if (plan->d_work != nullptr) QLT_HIP(hipFree(plan->d_work));
     plan->d_work = nullptr;
     plan->work_bytes = 0;
     plan->split_dtype = -1;
+    plan->split_mode = -1;
     QLT_HIP(hipMalloc(&plan->d_work, need));
     plan->work_bytes = need;
   }
-  if (plan->split_dtype != dtype) QLT_TRY(split_prepare_metadata(plan, dtype));
+  if (plan->split_dtype != dtype || plan->split_mode != mode) QLT_TRY(split_prepare_metadata(plan, dtype, mode));
   return QLT_OK;
 }
 
@@ -46,7 +47,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
   QLT_REQUIRE_INIT();
   QLT_CHECK_ARG(plan != nullptr, "qlt_contract_execute: null plan");
   QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_contract_execute: bad dtype");
-  QLT_CHECK_ARG(mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16, "qlt_contract_execute: bad mode");
+  QLT_CHECK_ARG(valid_mode(mode), "qlt_contract_execute: bad mode");
   QLT_CHECK_ARG(alpha != nullptr && beta != nullptr, "qlt_contract_execute: null alpha/beta");
   QLT_CHECK_ARG(plan->device == current_device(),
                 "qlt_contract_execute: plan was built on device " + std::to_string(plan->device));
@@ -54,12 +55,12 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
   QLT_CHECK_ARG(a != nullptr && b != nullptr && c != nullptr, "qlt_contract_execute: null data buffer");
   hipStream_t st = as_stream(stream);
   if (mode == QLT_MODE_NATIVE) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
-  if (split_profitable(plan, dtype) && plan->split_dtype != dtype) {
+  if (split_profitable(plan, dtype, mode) && (plan->split_dtype != dtype || plan->split_mode != mode)) {
     QLT_CHECK_ARG(!stream_capturing(st),
                   "qlt_contract_execute: split workspace not prepared; call qlt_plan_prepare before capture");
     QLT_TRY(qlt_plan_prepare(const_cast<qlt_plan *>(plan), dtype, mode));
   }
-  return launch_split(plan, dtype, a, b, c, alpha, beta, st);
+  return launch_split(plan, dtype, mode, a, b, c, alpha, beta, st);
 }
 
 }  // extern "C"

@@ -25,6 +25,20 @@
 //    Truncating at level 6 leaves a relative error of order 2^-45.
 // Complex128 is the 4-real-product form Cr = Ar Br - Ai Bi, Ci = Ar Bi + Ai Br
 // on the same slices, with one exponent per row/column over both parts.
+//
+// The int8 form (QLT_MODE_SPLIT_I8) keeps the scaling and replaces the digits:
+// the scaled value y is rounded to the integer Y = rint(y * 2^46) and written
+// in balanced base 255 as six digits d_0..d_5 in [-127, 127],
+//     Y = sum_s d_s 255^(5-s),
+// which negate without overflow (the complex real part needs -Ai).  The
+// GEMM runs v_mfma_i32_32x32x32_i8 on the 21 digit pairs with sa + sb <= 5
+// (the dropped pairs weigh at most 2^-46 of the row and column scales) and
+// sums every level L = sa + sb in its own int32 accumulator, which is exact
+// for 8192 k (a complex k adds at most 12 * 127^2 to a level).  A flush
+// recombines the six levels in FP64 by Horner's rule in 255, scales by
+// 255^5 2^(e_row + e_col - 92) and adds to the FP64 accumulator.  The int8
+// MFMA runs at twice the BF16 rate and a digit plane is half the bytes of a
+// BF16 plane.
 
 #include <algorithm>
 #include <cstring>
@@ -48,10 +62,14 @@ constexpr int TM = 64;        // tile rows
 constexpr int TN = 64;        // tile cols
 constexpr int RPAD = 64;      // packed rows are padded to a multiple of this
 constexpr int LDK = KC + 8;   // padded LDS row (bf16)
+constexpr int S8 = 6;         // int8 form: base-255 digits per element
+constexpr int KC8 = 32;       // int8 form: k per LDS stage (one int8 MFMA k-step)
+constexpr int kI8FlushK = 8192;   // int8 form: k between flushes of the int32 levels
 
-// Packed layout of one block (rows R_pad = round_up(R, 64), k_pad = round_up(K, 16)):
+// Packed layout of one block (rows R_pad = round_up(R, 64), k_pad = round_up(K, KC)):
 //   plane element (part, s, r, p) at
 //   (((part * S + s) * (k_pad / KC) + p / KC) * R_pad + r) * KC + p % KC
+// with S = 5, KC = 16 and BF16 elements, or S = 6, KC = 32 and int8 digits.
 struct PackMeta {
   int64_t plane_off;  // bf16 elements into the plane workspace
   int64_t exp_off;    // int32 elements into the exponent workspace
@@ -75,9 +93,32 @@ __device__ __forceinline__ uint16_t bf16_bits_exact(double v) {
   return static_cast<uint16_t>(__float_as_uint(f) >> 16);
 }
 
+// Balanced base-255 digits of y * 2^46 for |y| < 1 (slice 0 most significant).
+// All steps are exact in FP64: q is the integer nearest to Y / 255 (the
+// product's error stays far below the 1/510 distance of Y / 255 from a half
+// integer), and Y - 255 q is an integer of magnitude <= 127.
+__device__ __forceinline__ void i8_digits(double y, int8_t (&d)[S8]) {
+  double Y = rint(y * 70368744177664.0);   // 2^46
+#pragma unroll
+  for (int s = S8 - 1; s >= 0; --s) {
+    double q = rint(Y * (1.0 / 255.0));
+    double r = fma(-q, 255.0, Y);
+    if (r > 127.0) {
+      r -= 255.0;
+      q += 1.0;
+    } else if (r < -127.0) {
+      r += 255.0;
+      q -= 1.0;
+    }
+    d[s] = static_cast<int8_t>(static_cast<int>(r));
+    Y = q;
+  }
+}
+
 // Pack kernel: one workgroup of 256 threads per (side, block, 64-row group).
-// Pass 1 finds the max |x| of each row over all k; pass 2 splits and writes.
-template <bool CPLX>
+// Pass 1 finds the max |x| of each row over all k; pass 2 splits and writes
+// BF16 slices or (I8) int8 digits.
+template <bool CPLX, bool I8>
 __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items,
                                               const PackMeta *__restrict__ meta_a,
                                               const PackMeta *__restrict__ meta_b,
@@ -86,8 +127,9 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
                                               const int64_t *__restrict__ a_offs,
                                               const int64_t *__restrict__ b_offs, const GatherLayout L,
                                               const double *__restrict__ A, const double *__restrict__ B,
-                                              uint16_t *__restrict__ planes, int32_t *__restrict__ exps) {
+                                              void *__restrict__ planes_v, int32_t *__restrict__ exps) {
   constexpr int KP = CPLX ? 32 : 64;   // k per pass-2 chunk
+  constexpr int KCF = I8 ? KC8 : KC;   // k per packed chunk
   constexpr int PARTS = CPLX ? 2 : 1;
   __shared__ int32_t s_roff[64];
   __shared__ int32_t s_coff[64];
@@ -173,7 +215,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   __syncthreads();
 
   // ---- pass 2: split and write (including the zero padding) ----
-  const int nkc = pm.k_pad / KC;
+  const int nkc = pm.k_pad / KCF;
   for (int p0 = 0; p0 < pm.k_pad; p0 += KP) {
     if (tid < KP) {
       const int p = p0 + tid;
@@ -206,7 +248,44 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
     const int r = tid >> 2;
     const int pr = (tid & 3) * RUN;
     const int grow = it.r0 + r;
-    if (grow < pm.r_pad) {
+    if (I8 && grow < pm.r_pad) {
+      int8_t *planes = static_cast<int8_t *>(planes_v);
+      const double inv = ldexp(1.0, -s_exp[r]);
+#pragma unroll
+      for (int part = 0; part < PARTS; ++part) {
+        int8_t out[S8][RUN];
+#pragma unroll
+        for (int u = 0; u < RUN; ++u) {
+          int8_t d[S8];
+          i8_digits(s_tile[part][r][pr + u] * inv, d);   // the scaling is exact (power of two)
+#pragma unroll
+          for (int s = 0; s < S8; ++s) out[s][u] = d[s];
+        }
+        const int p = p0 + pr;
+        if (p < pm.k_pad) {
+          // RUN (8 or 16) consecutive k never cross a KC8 boundary
+#pragma unroll
+          for (int s = 0; s < S8; ++s) {
+            const int64_t idx =
+                pm.plane_off +
+                ((static_cast<int64_t>(part * S8 + s) * nkc + p / KC8) * pm.r_pad + grow) * KC8 + (p % KC8);
+            uint32_t w[RUN / 4];
+#pragma unroll
+            for (int u = 0; u < RUN / 4; ++u)
+              w[u] = static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u])) |
+                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 1])) << 8) |
+                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 2])) << 16) |
+                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 3])) << 24);
+            if constexpr (RUN == 16) {
+              *reinterpret_cast<v4u *>(planes + idx) = v4u{w[0], w[1], w[2], w[3]};
+            } else {
+              *reinterpret_cast<uint2 *>(planes + idx) = uint2{w[0], w[1]};
+            }
+          }
+        }
+      }
+    } else if (!I8 && grow < pm.r_pad) {
+      uint16_t *planes = static_cast<uint16_t *>(planes_v);
       const double inv = ldexp(1.0, -s_exp[r]);
 #pragma unroll
       for (int part = 0; part < PARTS; ++part) {
@@ -682,7 +761,384 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   }
 }
 
+// ---- int8 digit form ----
+
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+typedef int32_t v16i __attribute__((ext_vector_type(16)));
+
+// Bytewise negation of 16 int8 digits in [-127, 127] (no digit overflows):
+// per byte, (0x80 - (x & 0x7f)) cannot borrow, and xor-ing the complement of
+// x's sign bit gives -x.
+__device__ __forceinline__ v4i neg_digits(v4i x) {
+  const v4i h = v4i{static_cast<int32_t>(0x80808080u), static_cast<int32_t>(0x80808080u),
+                    static_cast<int32_t>(0x80808080u), static_cast<int32_t>(0x80808080u)};
+  return (h - (x & ~h)) ^ (~x & h);
+}
+
+// LDS position of the 16-byte half `half` of row `row` of a 32-byte digit row:
+// the halves of rows 4..7 (mod 8) are swapped, so the 16-byte reads of eight
+// consecutive rows cover every bank once.
+__device__ __forceinline__ int i8_half(int row, int half) { return 16 * (half ^ ((row >> 2) & 1)); }
+
+// Recombines the six int32 digit levels of accumulator element r in FP64 and
+// adds 255^5 2^(e - 92) times the result to acc.
+__device__ __forceinline__ double i8_level_sum(const v16i (&g)[S8], int r) {
+  double sum = static_cast<double>(g[0][r]);
+#pragma unroll
+  for (int l = 1; l < S8; ++l) sum = fma(sum, 255.0, static_cast<double>(g[l][r]));
+  return sum;
+}
+constexpr double k255p5 = 1078203909375.0;   // 255^5, exact in FP64
+
+// Real int8 split GEMM: one 256-thread workgroup per 64 x 64 output tile,
+// 2 x 2 waves of 32 x 32, the 21 digit pairs of a 32-k chunk as one
+// v_mfma_i32_32x32x32_i8 each, with the software pipeline of
+// k_split_gemm_f64 (the next chunk's digits in flight while the MFMAs run).
+__global__ __launch_bounds__(256) void k_split_i8_f64(
+    const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
+    const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
+    double alpha, double beta) {
+  constexpr int PIECES = S8 * 128;                 // 16-byte pieces per operand per k-chunk
+  constexpr int PER_THREAD = PIECES / 256;
+  static_assert(PIECES % 256 == 0, "pieces per thread");
+  __shared__ __attribute__((aligned(16))) int8_t As[2][S8][TM][KC8];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][S8][TN][KC8];
+  __shared__ int32_t s_ea[2][TM];
+
+  const Tile tile = tiles[split_xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
+  const Seg seg = segs[tile.seg];
+  const int i0 = tile.tm * TM, j0 = tile.tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int lrow = lane & 31, lh = lane >> 5;
+  const bool active = (i0 + wm < seg.m) && (j0 + wn < seg.n);
+
+  double acc64[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+  v16i g[S8];
+#pragma unroll
+  for (int l = 0; l < S8; ++l) g[l] = v16i{};
+
+  v4u pa[PER_THREAD], pb[PER_THREAD];
+  int32_t ea_next = 0;
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc) {
+    const int nkc = ma.k_pad / KC8;
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + 256 * u;
+      const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+      const int64_t ia = ma.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * ma.r_pad + i0 + rr) * KC8 + half * 16;
+      const int64_t ib = mb.plane_off + ((static_cast<int64_t>(ps) * nkc + kc) * mb.r_pad + j0 + rr) * KC8 + half * 16;
+      pa[u] = *reinterpret_cast<const v4u *>(planes + ia);
+      pb[u] = *reinterpret_cast<const v4u *>(planes + ib);
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + 256 * u;
+      const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+      *reinterpret_cast<v4u *>(&As[buf][ps][rr][i8_half(rr, half)]) = pa[u];
+      *reinterpret_cast<v4u *>(&Bs[buf][ps][rr][i8_half(rr, half)]) = pb[u];
+    }
+  };
+
+  int t = seg.t0, kc = 0;
+  PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
+  int nkc = ma.k_pad / KC8;
+  int32_t eb = exps[mb.exp_off + j0 + wn + lrow];
+  fetch(ma, mb, 0);
+  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + i0 + tid];
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+
+  while (t < seg.t1) {
+    int t2 = t, kc2 = kc + 1;
+    if (kc2 == nkc) {
+      t2 = t + 1;
+      kc2 = 0;
+    }
+    const bool has_next = t2 < seg.t1;
+    PackMeta ma2 = ma, mb2 = mb;
+    int32_t eb2 = eb;
+    if (has_next) {
+      if (t2 != t) {
+        const ExecTask e2 = exec[t2];
+        ma2 = meta_a[e2.a_blk];
+        mb2 = meta_b[e2.b_blk];
+        eb2 = exps[mb2.exp_off + j0 + wn + lrow];
+        if (tid < TM) ea_next = exps[ma2.exp_off + i0 + tid];
+      }
+      fetch(ma2, mb2, kc2);
+    }
+    if (active) {
+      const int brow = wn + lrow, arow = wm + lrow;
+      v4i bfr[S8];
+#pragma unroll
+      for (int s2 = 0; s2 < S8; ++s2) bfr[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][s2][brow][i8_half(brow, lh)]);
+#pragma unroll
+      for (int sa = 0; sa < S8; ++sa) {
+        const v4i ar = *reinterpret_cast<const v4i *>(&As[buf][sa][arow][i8_half(arow, lh)]);
+#pragma unroll
+        for (int sb = 0; sb < S8 - sa; ++sb)
+          g[sa + sb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar, bfr[sb], g[sa + sb], 0, 0, 0);
+      }
+      const int kk = (kc + 1) * KC8;                // k since the task start
+      if (kc == nkc - 1 || kk % kI8FlushK == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = wm + 8 * (r >> 2) + 4 * lh + (r & 3);
+          acc64[r] = fma(ldexp(i8_level_sum(g, r), s_ea[buf][row] + eb - 92), k255p5, acc64[r]);
+        }
+#pragma unroll
+        for (int l = 0; l < S8; ++l) g[l] = v16i{};
+      }
+    }
+    if (has_next) {
+      stage(buf ^ 1);
+      if (tid < TM) s_ea[buf ^ 1][tid] = (t2 != t) ? ea_next : s_ea[buf][tid];
+    }
+    __syncthreads();
+    if (t2 != t) {
+      ma = ma2;
+      mb = mb2;
+      nkc = ma.k_pad / KC8;
+      eb = eb2;
+    }
+    t = t2;
+    kc = kc2;
+    buf ^= 1;
+  }
+
+  const int col = j0 + wn + lrow;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
+    if (row < seg.m && col < seg.n) {
+      const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
+      double y = alpha * acc64[r];
+      if (beta != 0.0) y += beta * C[idx];
+      C[idx] = y;
+    }
+  }
+}
+
+// Complex int8 split GEMM: the persistent structure of k_split_gemm_c2 (one
+// 512-thread workgroup per CU walking tiles blockIdx.x, blockIdx.x + gridDim.x,
+// ...; waves 0-3 the real part ar*br + (-ai)*bi, waves 4-7 the imaginary part
+// ar*bi + ai*br of the same 2 x 2 arrangement of 32 x 32 blocks; the pipeline
+// crosses task and tile boundaries; the imaginary parts of a finished tile go
+// to the real-part waves through an LDS exchange area) on int8 digits.
+__global__ __launch_bounds__(512) void k_split_i8_c2(
+    const Tile *__restrict__ tiles, int32_t n_tiles, const Seg *__restrict__ segs,
+    const ExecTask *__restrict__ exec, const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
+    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int PARTS = 2;
+  constexpr int PIECES = PARTS * S8 * 128;          // 16-byte pieces per operand per k-chunk
+  constexpr int NT = 512;
+  constexpr int PER_THREAD = PIECES / NT;
+  static_assert(PIECES % NT == 0, "pieces per thread");
+  __shared__ __attribute__((aligned(16))) int8_t As[2][PARTS][S8][TM][KC8];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][PARTS][S8][TN][KC8];
+  __shared__ int32_t s_ea[2][TM];
+  __shared__ double xch[4][16][64];                 // imaginary parts of a finished tile
+
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ws = w & 3, comp = w >> 2;             // spatial block, output component
+  const int wm = (ws >> 1) * 32, wn = (ws & 1) * 32;
+  const int lrow = lane & 31, lh = lane >> 5;
+  const int G = static_cast<int>(gridDim.x);
+
+  struct TileRef {
+    int i0, j0;
+    Seg seg;
+  };
+  auto tile_at = [&](int bidx) __attribute__((always_inline)) {
+    TileRef r;
+    const Tile tl = tiles[split_xcd_tile(bidx, n_tiles)];
+    r.seg = segs[tl.seg];
+    r.i0 = tl.tm * TM;
+    r.j0 = tl.tn * TN;
+    return r;
+  };
+
+  double acc64[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+  v16i g[S8];
+#pragma unroll
+  for (int l = 0; l < S8; ++l) g[l] = v16i{};
+
+  v4u pa[PER_THREAD], pb[PER_THREAD];
+  int32_t ea_next = 0;
+  // byte offsets of a thread's pieces within a task's k-chunk 0 (a block's
+  // digit planes stay below 2^31 bytes), recomputed when the task changes
+  uint32_t oa[PER_THREAD], ob[PER_THREAD];
+  auto task_offsets = [&](const PackMeta &ma, const PackMeta &mb) __attribute__((always_inline)) {
+    const int nkc = ma.k_pad / KC8;
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + NT * u;
+      const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+      oa[u] = static_cast<uint32_t>((ps * nkc * ma.r_pad + rr) * KC8 + half * 16);
+      ob[u] = static_cast<uint32_t>((ps * nkc * mb.r_pad + rr) * KC8 + half * 16);
+    }
+  };
+  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc, int i0, int j0) __attribute__((always_inline)) {
+    const int8_t *ba = planes + ma.plane_off + (static_cast<int64_t>(kc) * ma.r_pad + i0) * KC8;
+    const int8_t *bb = planes + mb.plane_off + (static_cast<int64_t>(kc) * mb.r_pad + j0) * KC8;
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      pa[u] = *reinterpret_cast<const v4u *>(ba + oa[u]);
+      pb[u] = *reinterpret_cast<const v4u *>(bb + ob[u]);
+    }
+  };
+  auto stage = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int c = tid + NT * u;
+      const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
+      *reinterpret_cast<v4u *>(&As[buf][ps / S8][ps % S8][rr][i8_half(rr, half)]) = pa[u];
+      *reinterpret_cast<v4u *>(&Bs[buf][ps / S8][ps % S8][rr][i8_half(rr, half)]) = pb[u];
+    }
+  };
+
+  int bt = static_cast<int>(blockIdx.x);
+  TileRef cur = tile_at(bt);
+  int t = cur.seg.t0, kc = 0;
+  PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
+  int nkc = ma.k_pad / KC8;
+  int32_t eb = exps[mb.exp_off + cur.j0 + wn + lrow];
+  task_offsets(ma, mb);
+  fetch(ma, mb, 0, cur.i0, cur.j0);
+  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + cur.i0 + tid];
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+
+  while (true) {
+    int t2 = t, kc2 = kc + 1;
+    bool new_task = false;
+    if (kc2 == nkc) {
+      t2 = t + 1;
+      kc2 = 0;
+      new_task = true;
+    }
+    const bool tile_end = t2 >= cur.seg.t1;
+    const int bt2 = tile_end ? bt + G : bt;
+    const bool has_next = bt2 < n_tiles;
+    TileRef nxt = cur;
+    PackMeta ma2 = ma, mb2 = mb;
+    int32_t eb2 = eb;
+    if (has_next) {
+      if (tile_end) {
+        nxt = tile_at(bt2);
+        t2 = nxt.seg.t0;
+      }
+      if (new_task) {
+        const ExecTask e2 = exec[t2];
+        ma2 = meta_a[e2.a_blk];
+        mb2 = meta_b[e2.b_blk];
+        eb2 = exps[mb2.exp_off + nxt.j0 + wn + lrow];
+        if (tid < TM) ea_next = exps[ma2.exp_off + nxt.i0 + tid];
+        task_offsets(ma2, mb2);
+      }
+      fetch(ma2, mb2, kc2, nxt.i0, nxt.j0);
+    }
+    const bool active = (cur.i0 + wm < cur.seg.m) && (cur.j0 + wn < cur.seg.n);
+    if (active) {
+      const int brow = wn + lrow, arow = wm + lrow;
+      v4i b0[S8], b1[S8];
+#pragma unroll
+      for (int s2 = 0; s2 < S8; ++s2) {
+        b0[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][comp][s2][brow][i8_half(brow, lh)]);
+        b1[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][1 - comp][s2][brow][i8_half(brow, lh)]);
+      }
+#pragma unroll
+      for (int sa = 0; sa < S8; ++sa) {
+        const v4i ar = *reinterpret_cast<const v4i *>(&As[buf][0][sa][arow][i8_half(arow, lh)]);
+        v4i ai = *reinterpret_cast<const v4i *>(&As[buf][1][sa][arow][i8_half(arow, lh)]);
+        if (comp == 0) ai = neg_digits(ai);
+#pragma unroll
+        for (int sb = 0; sb < S8 - sa; ++sb) {
+          g[sa + sb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar, b0[sb], g[sa + sb], 0, 0, 0);
+          g[sa + sb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai, b1[sb], g[sa + sb], 0, 0, 0);
+        }
+      }
+      const int kk = (kc + 1) * KC8;                // k since the task start
+      if (kc == nkc - 1 || kk % kI8FlushK == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = wm + 8 * (r >> 2) + 4 * lh + (r & 3);
+          acc64[r] = fma(ldexp(i8_level_sum(g, r), s_ea[buf][row] + eb - 92), k255p5, acc64[r]);
+        }
+#pragma unroll
+        for (int l = 0; l < S8; ++l) g[l] = v16i{};
+      }
+    }
+    if (tile_end) {
+      if (comp == 1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xch[ws][r][lane] = acc64[r];
+      }
+      __syncthreads();
+      if (comp == 0 && active) {
+        const int col = cur.j0 + wn + lrow;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
+          if (row < cur.seg.m && col < cur.seg.n) {
+            const int64_t idx = cur.seg.c_off + static_cast<int64_t>(row) * cur.seg.n + col;
+            const double xr = acc64[r], xi = xch[ws][r][lane];
+            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+            if (use_beta) {
+              const double2 o = *cp;
+              yr += beta_r * o.x - beta_i * o.y;
+              yi += beta_r * o.y + beta_i * o.x;
+            }
+            *cp = double2{yr, yi};
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+    }
+    if (has_next) {
+      stage(buf ^ 1);
+      if (tid < TM) s_ea[buf ^ 1][tid] = new_task ? ea_next : s_ea[buf][tid];
+    }
+    __syncthreads();
+    if (!has_next) break;
+    if (tile_end) {
+      cur = nxt;
+      bt = bt2;
+    }
+    if (new_task) {
+      ma = ma2;
+      mb = mb2;
+      nkc = ma.k_pad / KC8;
+      eb = eb2;
+    }
+    t = t2;
+    kc = kc2;
+    buf ^= 1;
+  }
+}
+
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Digit format of a split mode: slices per element, k per packed chunk,
+// bytes per plane element.
+struct SplitFmt {
+  int slices, kc, elem_bytes;
+};
+inline SplitFmt split_fmt(int32_t mode) { return mode == QLT_MODE_SPLIT_I8 ? SplitFmt{S8, KC8, 1} : SplitFmt{S, KC, 2}; }
 
 // Host-side layout of the split workspace of a plan.
 struct SplitLayout {
@@ -695,9 +1151,10 @@ struct SplitLayout {
          off_planes = 0, off_tiles = 0, total = 0;
 };
 
-SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
+SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
   SplitLayout L;
   const int parts = (dtype == QLT_C128) ? 2 : 1;
+  const SplitFmt fmt = split_fmt(mode);
   L.meta_a.assign(plan->n_a_blks, PackMeta{0, 0, 0, 0, 0, 0});
   L.meta_b.assign(plan->n_b_blks, PackMeta{0, 0, 0, 0, 0, 0});
   std::vector<char> used_a(plan->n_a_blks, 0), used_b(plan->n_b_blks, 0);
@@ -712,7 +1169,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
         m.rows = s.m;
         m.cols = e.k;
         m.r_pad = static_cast<int32_t>(round_up(s.m, RPAD));
-        m.k_pad = static_cast<int32_t>(round_up(e.k, KC));
+        m.k_pad = static_cast<int32_t>(round_up(e.k, fmt.kc));
       }
       if (!used_b[e.b_blk]) {
         used_b[e.b_blk] = 1;
@@ -720,7 +1177,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
         m.rows = s.n;
         m.cols = e.k;
         m.r_pad = static_cast<int32_t>(round_up(s.n, RPAD));
-        m.k_pad = static_cast<int32_t>(round_up(e.k, KC));
+        m.k_pad = static_cast<int32_t>(round_up(e.k, fmt.kc));
       }
     }
   }
@@ -730,7 +1187,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
       PackMeta &m = meta[i];
       m.plane_off = L.plane_elems;
       m.exp_off = L.exp_elems;
-      L.plane_elems += static_cast<int64_t>(parts) * S * m.r_pad * m.k_pad;
+      L.plane_elems += static_cast<int64_t>(parts) * fmt.slices * m.r_pad * m.k_pad;
       L.exp_elems += m.r_pad;
       for (int32_t r0 = 0; r0 < m.r_pad; r0 += 64)
         L.items.push_back(PackItem{side, static_cast<int32_t>(i), r0, 0});
@@ -768,7 +1225,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype) {
   L.off_exps = off;
   off = align(off + sizeof(int32_t) * L.exp_elems);
   L.off_planes = off;
-  off = align(off + sizeof(uint16_t) * L.plane_elems);
+  off = align(off + static_cast<size_t>(fmt.elem_bytes) * L.plane_elems);
   L.off_tiles = off;
   off = align(off + sizeof(Tile) * L.tiles.size());
   L.total = off;
@@ -796,21 +1253,22 @@ static double g_split_min_mean_k = 48.0;
 // operands are large relative to its flops is cheaper natively.
 constexpr double kSplitMinFlopsPerByte = 96.0;
 
-bool split_profitable(const qlt_plan *plan, int32_t dtype) {
+bool split_profitable(const qlt_plan *plan, int32_t dtype, int32_t mode) {
   int64_t mfma_tiles = 0;
   for (int c = 0; c < kNumMfmaClasses; ++c) mfma_tiles += plan->cls_tiles[c];
   if (mfma_tiles == 0) return false;
   double w = 0.0, wk = 0.0, elems = 0.0;
   std::vector<char> seen_a(plan->n_a_blks, 0), seen_b(plan->n_b_blks, 0);
   const int64_t parts = (dtype == QLT_C128) ? 2 : 1;
+  const SplitFmt fmt = split_fmt(mode);
   for (const Seg &s : plan->h_segs) {
     if (!is_mfma_class(s.cls)) continue;
     for (int32_t t = s.t0; t < s.t1; ++t) {
       const ExecTask &e = plan->h_exec[t];
       // the split kernels address a block's slice planes with 32-bit offsets
-      const int64_t kp = round_up(e.k, KC);
-      if (parts * S * round_up(s.m, RPAD) * kp >= (int64_t{1} << 31) ||
-          parts * S * round_up(s.n, RPAD) * kp >= (int64_t{1} << 31))
+      const int64_t kp = round_up(e.k, fmt.kc);
+      if (parts * fmt.slices * round_up(s.m, RPAD) * kp >= (int64_t{1} << 31) ||
+          parts * fmt.slices * round_up(s.n, RPAD) * kp >= (int64_t{1} << 31))
         return false;
       const double k = e.k;
       const double f = static_cast<double>(s.m) * s.n * k;
@@ -833,19 +1291,19 @@ bool split_profitable(const qlt_plan *plan, int32_t dtype) {
   return flops / bytes >= kSplitMinFlopsPerByte;
 }
 
-qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, size_t *bytes) {
-  if (!split_profitable(plan, dtype)) {
+qlt_status split_workspace_bytes(const qlt_plan *plan, int32_t dtype, int32_t mode, size_t *bytes) {
+  if (!split_profitable(plan, dtype, mode)) {
     *bytes = 0;
     return QLT_OK;
   }
-  *bytes = compute_layout(plan, dtype).total;
+  *bytes = compute_layout(plan, dtype, mode).total;
   return QLT_OK;
 }
 
-qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
+qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype, int32_t mode) {
   // Writes the pack metadata into the (already allocated) workspace.
-  if (!split_profitable(plan, dtype)) return QLT_OK;
-  SplitLayout L = compute_layout(plan, dtype);
+  if (!split_profitable(plan, dtype, mode)) return QLT_OK;
+  SplitLayout L = compute_layout(plan, dtype, mode);
   if (plan->work_bytes < L.total) return fail(QLT_ERR_INTERNAL, "split workspace too small");
   char *base = static_cast<char *>(plan->d_work);
   std::vector<int64_t> aoffs(plan->n_a_blks, 0), boffs(plan->n_b_blks, 0);
@@ -863,6 +1321,7 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
   QLT_HIP(hipMemcpy(base + L.off_boffs, boffs.data(), sizeof(int64_t) * boffs.size(), hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_tiles, L.tiles.data(), sizeof(Tile) * L.tiles.size(), hipMemcpyHostToDevice));
   plan->split_dtype = dtype;
+  plan->split_mode = mode;
   plan->split_items = static_cast<int64_t>(L.items.size());
   plan->split_tiles = static_cast<int64_t>(L.tiles.size());
   const size_t offs[8] = {L.off_meta_a, L.off_meta_b, L.off_items, L.off_aoffs, L.off_boffs, L.off_exps,
@@ -871,11 +1330,12 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype) {
   return QLT_OK;
 }
 
-qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
+qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const void *a, const void *b, void *c,
                         const double *alpha, const double *beta, hipStream_t st) {
-  if (!split_profitable(plan, dtype)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
-  if (plan->split_dtype != dtype)
-    return fail(QLT_ERR_INVALID_ARG, "split workspace was prepared for another dtype; call qlt_plan_prepare");
+  if (!split_profitable(plan, dtype, mode)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
+  if (plan->split_dtype != dtype || plan->split_mode != mode)
+    return fail(QLT_ERR_INVALID_ARG,
+                "split workspace was prepared for another dtype or mode; call qlt_plan_prepare");
   if (plan->d_work == nullptr) return fail(QLT_ERR_INTERNAL, "split workspace missing");
   char *base = static_cast<char *>(plan->d_work);
   const size_t *so = plan->split_off;
@@ -885,7 +1345,7 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   const auto *aoffs = reinterpret_cast<const int64_t *>(base + so[3]);
   const auto *boffs = reinterpret_cast<const int64_t *>(base + so[4]);
   auto *exps = reinterpret_cast<int32_t *>(base + so[5]);
-  auto *planes = reinterpret_cast<uint16_t *>(base + so[6]);
+  void *planes = base + so[6];
   const auto *stiles = reinterpret_cast<const Tile *>(base + so[7]);
   const unsigned n_items = static_cast<unsigned>(plan->split_items);
   const unsigned n_mfma = static_cast<unsigned>(plan->split_tiles);
@@ -900,29 +1360,45 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, const void *a, cons
   for (int32_t cls = 0; cls < kNumTileClasses; ++cls) (is_mfma_class(cls) ? mfma_mkn : rows_mkn) += plan->cls_mkn[cls];
   const bool probe_split = probe_armed(st) && mfma_mkn >= rows_mkn;
   const bool cplx = dtype == QLT_C128;
-  if (dtype == QLT_F64) {
-    hipLaunchKernelGGL(k_pack<false>, dim3(n_items), blk, 0, st, items, meta_a,
-                       meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
-                       static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+  const bool i8 = mode == QLT_MODE_SPLIT_I8;
+  const auto *planes16 = static_cast<const uint16_t *>(planes);
+  const auto *planes8 = static_cast<const int8_t *>(planes);
+  auto *cd = static_cast<double *>(c);
+  const auto *ad = static_cast<const double *>(a);
+  const auto *bd = static_cast<const double *>(b);
+  // the complex kernels are persistent: one workgroup per CU (their LDS allows one)
+  const unsigned grid = std::min<unsigned>(n_mfma, static_cast<unsigned>(device_cu_count()));
+  if (!cplx && !i8) {
+    hipLaunchKernelGGL((k_pack<false, false>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
+                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
     if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_gemm_f64, dim3(n_mfma), blk, 0, st,
-                       stiles, plan->d_segs, plan->d_exec, meta_a, meta_b, planes, exps,
-                       static_cast<double *>(c), ar, br);
+    hipLaunchKernelGGL(k_split_gemm_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
+                       meta_b, planes16, exps, cd, ar, br);
+  } else if (cplx && !i8) {
+    hipLaunchKernelGGL((k_pack<true, false>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
+                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    if (probe_split) probe_begin(st);
+    hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
+                       plan->d_segs, plan->d_exec, meta_a, meta_b, planes16, exps, cd, ar, ai, br, bi);
+  } else if (!cplx) {
+    hipLaunchKernelGGL((k_pack<false, true>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
+                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    if (probe_split) probe_begin(st);
+    hipLaunchKernelGGL(k_split_i8_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
+                       meta_b, planes8, exps, cd, ar, br);
   } else {
-    hipLaunchKernelGGL(k_pack<true>, dim3(n_items), blk, 0, st, items, meta_a,
-                       meta_b, plan->d_ga, plan->d_gb, aoffs, boffs, plan->layout,
-                       static_cast<const double *>(a), static_cast<const double *>(b), planes, exps);
+    hipLaunchKernelGGL((k_pack<true, true>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
+                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
     if (probe_split) probe_begin(st);
-    // persistent: one workgroup per CU (the kernel's LDS allows one)
-    const unsigned grid = std::min<unsigned>(n_mfma, static_cast<unsigned>(device_cu_count()));
-    hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st,
-                       stiles, static_cast<int32_t>(n_mfma), plan->d_segs, plan->d_exec, meta_a, meta_b, planes,
-                       exps, static_cast<double *>(c), ar, ai, br, bi);
+    hipLaunchKernelGGL(k_split_i8_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
+                       plan->d_segs, plan->d_exec, meta_a, meta_b, planes8, exps, cd, ar, ai, br, bi);
   }
   if (probe_split) {
     double elems = 0.0;
     for (int32_t cls = 0; cls < kNumMfmaClasses; ++cls) elems += plan->cls_elems[cls];
-    probe_end(st, cplx ? "k_split_gemm_c2 c128" : "k_split_gemm_f64 f64", (cplx ? 8.0 : 2.0) * mfma_mkn,
+    const char *name = i8 ? (cplx ? "k_split_i8_c2 c128" : "k_split_i8_f64 f64")
+                          : (cplx ? "k_split_gemm_c2 c128" : "k_split_gemm_f64 f64");
+    probe_end(st, name, (cplx ? 8.0 : 2.0) * mfma_mkn,
               (cplx ? 16.0 : 8.0) * elems);
   }
   QLT_HIP(hipGetLastError());

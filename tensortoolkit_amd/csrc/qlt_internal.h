@@ -144,6 +144,9 @@ constexpr int32_t kNumTileClasses = 6;
 __host__ __device__ constexpr int32_t mfma_tile_m(int32_t cls) { return (cls == 1 || cls == 3) ? 32 : 64; }
 __host__ __device__ constexpr int32_t mfma_tile_n(int32_t cls) { return (cls == 2 || cls == 3) ? 32 : 64; }
 __host__ __device__ constexpr bool is_mfma_class(int32_t cls) { return cls < kNumMfmaClasses; }
+constexpr bool valid_mode(int32_t mode) {
+  return mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16 || mode == QLT_MODE_SPLIT_I8;
+}
 
 // 32 along a dimension when it cuts the padded extent by at least 10%.
 __host__ __device__ constexpr bool prefer_32(int64_t x) {
@@ -299,6 +302,7 @@ struct qlt_plan {
   void *d_work = nullptr;
   size_t work_bytes = 0;
   int32_t split_dtype = -1;             // dtype the split metadata was written for
+  int32_t split_mode = -1;              // split mode (digit format) it was written for
   int64_t split_items = 0;              // pack work items
   size_t split_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // workspace sub-buffer offsets
   int64_t split_tiles = 0;              // 64 x 64 tiles of the split kernel

@@ -1,4 +1,5 @@
-"""GPU parity of the split-BF16 (FP64-emulating) path against the FP64 oracle.
+"""GPU parity of the split (FP64-emulating) paths, BF16 slices and int8
+digits, against the FP64 oracle.
 
 Every output block must be within 1e-12 relative Frobenius error of the
 oracle, and the plan is the same plan as the native
@@ -16,7 +17,11 @@ from tests.test_contract_gpu import gpu_contract
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
-SPLIT = capi.QLT_MODE_SPLIT_BF16
+
+
+@pytest.fixture(params=[capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8], ids=["bf16", "i8"])
+def split_mode(request):
+    return request.param
 
 
 @pytest.fixture
@@ -34,19 +39,19 @@ def _say(msg):
 
 @pytest.mark.parametrize("seed", range(10))
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
-def test_split_random(forced_split, seed, dtype):
+def test_split_random(forced_split, split_mode, seed, dtype):
     rng = np.random.default_rng(9000 + seed)
     a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1 + seed % 2, max_rank=3, max_sectors=3,
                                    max_deg=40, min_deg=8, allow_scalar=False)
-    _say(f"case: split seed={seed} {dtype} ranks={a.rank},{b.rank} axes={a_ax},{b_ax}")
-    plan, c = gpu_contract(a, b, a_ax, b_ax, mode=SPLIT)
+    _say(f"case: split mode={split_mode} seed={seed} {dtype} ranks={a.rank},{b.rank} axes={a_ax},{b_ax}")
+    plan, c = gpu_contract(a, b, a_ax, b_ax, mode=split_mode)
     ref = O.OraclePlan(a, b, a_ax, b_ax)
     err = per_block_rel_err(c, ref.execute())
     assert err <= TOL, err
 
 
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
-def test_split_long_k_many_flushes(forced_split, dtype):
+def test_split_long_k_many_flushes(forced_split, split_mode, dtype):
     """k = 3 * 200 = 600 spans many FP32 flush intervals; ragged m, n."""
     ix_l = Index(((0,), (1,)), (70, 131), IN)
     ix_r = Index(((0,), (1,)), (200, 67), OUT)
@@ -55,14 +60,35 @@ def test_split_long_k_many_flushes(forced_split, dtype):
     b = BlockSparseTensor.allowed_blocks([ix_r.inverse(), ix_p, ix_l.inverse()], (0,), dtype).fill_uniform(22)
     for axes in (([2, 1], [0, 1]), ([0], [2])):
         _say(f"case: split long-k {dtype} axes={axes}")
-        plan, c = gpu_contract(a, b, *axes, mode=SPLIT)
+        plan, c = gpu_contract(a, b, *axes, mode=split_mode)
         ref = O.OraclePlan(a, b, *axes)
         err = per_block_rel_err(c, ref.execute())
         assert err <= TOL, (axes, err)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
-def test_split_wide_dynamic_range(forced_split, dtype):
+def test_split_k_beyond_int8_flush_interval(forced_split, split_mode, dtype):
+    """k = 8500 crosses the int8 form's 8192-k flush of its int32 digit
+    levels; entries in [0.5, 1) of one sign make every level sum grow as fast
+    as it can, so a missed flush would overflow int32."""
+    ix_m = Index(((0,),), (70,), IN)
+    ix_k = Index(((0,),), (8500,), OUT)
+    ix_n = Index(((0,),), (40,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix_m, ix_k], (0,), dtype).fill_uniform(51)
+    b = BlockSparseTensor.allowed_blocks([ix_k.inverse(), ix_n], (0,), dtype).fill_uniform(52)
+    if dtype == "c128":
+        a.data = 0.5 + 0.5 * np.abs(a.data.real) + 1j * (0.5 + 0.5 * np.abs(a.data.imag))
+        b.data = 0.5 + 0.5 * np.abs(b.data.real) + 1j * (0.5 + 0.5 * np.abs(b.data.imag))
+    else:
+        a.data = 0.5 + 0.5 * np.abs(a.data)
+        b.data = 0.5 + 0.5 * np.abs(b.data)
+    plan, c = gpu_contract(a, b, [1], [0], mode=split_mode)
+    exp = O.OraclePlan(a, b, [1], [0]).execute()
+    assert rel_frobenius(c.data, exp) <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_wide_dynamic_range(forced_split, split_mode, dtype):
     """Rows of A and columns of B scaled by 2^-30 .. 2^30: the per-row and
     per-column power-of-two scaling keeps every output block accurate."""
     ix_m = Index(((0,),), (40,), IN)
@@ -74,7 +100,7 @@ def test_split_wide_dynamic_range(forced_split, dtype):
     cs = 2.0 ** np.linspace(30, -30, 50)
     a.data = (a.data.reshape(40, 90) * rs[:, None]).ravel()
     b.data = (b.data.reshape(90, 50) * cs[None, :]).ravel()
-    plan, c = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    plan, c = gpu_contract(a, b, [1], [0], mode=split_mode)
     exp = O.OraclePlan(a, b, [1], [0]).execute()
     # Each output row (and column) is an inner product against one scaled row
     # of A (column of B), so with per-row / per-column scaling every row and
@@ -90,19 +116,19 @@ def test_split_wide_dynamic_range(forced_split, dtype):
     assert rel_frobenius(c.data, exp) <= TOL
 
 
-def test_split_zero_blocks_and_rows(forced_split):
+def test_split_zero_blocks_and_rows(forced_split, split_mode):
     ix = Index(((0,),), (70,), OUT)
     a = BlockSparseTensor.allowed_blocks([ix.inverse(), ix], (0,)).fill_uniform(41)
     b = BlockSparseTensor.allowed_blocks([ix.inverse(), ix], (0,)).fill_uniform(42)
     a.data = a.data.copy()
     a.data.reshape(70, 70)[5:17, :] = 0.0      # zero rows take exponent 0
-    plan, c = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    plan, c = gpu_contract(a, b, [1], [0], mode=split_mode)
     exp = O.OraclePlan(a, b, [1], [0]).execute()
     assert rel_frobenius(c.data, exp) <= TOL
     assert np.all(c.data.reshape(70, 70)[5:17, :] == 0.0)
 
 
-def test_split_beta_and_alpha(forced_split):
+def test_split_beta_and_alpha(forced_split, split_mode):
     rng = np.random.default_rng(5)
     a, b, a_ax, b_ax = random_case(rng, dtype="c128", allow_scalar=False, max_deg=8)
     plan = ContractionPlan(a, b, a_ax, b_ax)
@@ -111,21 +137,21 @@ def test_split_beta_and_alpha(forced_split):
     da, db = capi.DeviceBuffer.from_numpy(a.data), capi.DeviceBuffer.from_numpy(b.data)
     dc = capi.DeviceBuffer.from_numpy(init.astype(np.complex128))
     alpha, beta = 0.5 - 1.5j, 2.0 + 0.25j
-    plan.execute("c128", SPLIT, da.ptr, db.ptr, dc.ptr, alpha=alpha, beta=beta)
+    plan.execute("c128", split_mode, da.ptr, db.ptr, dc.ptr, alpha=alpha, beta=beta)
     got = dc.to_numpy(np.complex128)
     exp = alpha * O.OraclePlan(a, b, a_ax, b_ax).execute() + beta * init
     assert rel_frobenius(got, exp) <= TOL
 
 
 @pytest.mark.parametrize("u1u1,dtype", [(False, "f64"), (True, "c128")])
-def test_split_small_matvec_chain(forced_split, u1u1, dtype):
+def test_split_small_matvec_chain(forced_split, split_mode, u1u1, dtype):
     mv = W.small_matvec(D=64 if u1u1 else 96, dtype=dtype, u1u1=u1u1)
     _say(f"case: split small matvec {mv.name}")
     x = x_ref = mv.psi
     for step, (ops, axes) in enumerate(zip((mv.L, mv.W, mv.W, mv.R), W.MATVEC_AXES)):
         first, second = (ops, x) if step == 0 else (x, ops)
         first_ref, second_ref = (ops, x_ref) if step == 0 else (x_ref, ops)
-        plan, c = gpu_contract(first, second, list(axes[0]), list(axes[1]), mode=SPLIT)
+        plan, c = gpu_contract(first, second, list(axes[0]), list(axes[1]), mode=split_mode)
         ref = O.OraclePlan(first_ref, second_ref, list(axes[0]), list(axes[1]))
         c_ref = ref.output_tensor(dtype)
         c_ref.data = ref.execute()
@@ -134,7 +160,7 @@ def test_split_small_matvec_chain(forced_split, u1u1, dtype):
         x, x_ref = c, c_ref
 
 
-def test_split_default_dispatch_keeps_low_k_native(qlt):
+def test_split_default_dispatch_keeps_low_k_native(qlt, split_mode):
     """With the default threshold a k=2 contraction runs natively in split
     mode (bit-identical to the native result)."""
     ix = Index(((0,), (1,)), (30, 40), IN)
@@ -142,5 +168,5 @@ def test_split_default_dispatch_keeps_low_k_native(qlt):
     a = BlockSparseTensor.allowed_blocks([ix, p], (0,)).fill_uniform(1)
     b = BlockSparseTensor.allowed_blocks([p.inverse(), ix.inverse()], (0,)).fill_uniform(2)
     _, c_native = gpu_contract(a, b, [1], [0])
-    _, c_split = gpu_contract(a, b, [1], [0], mode=SPLIT)
+    _, c_split = gpu_contract(a, b, [1], [0], mode=split_mode)
     np.testing.assert_array_equal(c_native.data, c_split.data)
