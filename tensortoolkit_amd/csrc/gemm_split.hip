@@ -83,7 +83,7 @@ struct PackItem {
   int32_t side;       // 0: A block (rows = saved, cols = contracted); 1: B block transposed
   int32_t blk;        // block position
   int32_t r0;         // first row of this 64-row group
-  int32_t pad;
+  int32_t k0;         // first k of this chunk (digit phase)
 };
 
 __device__ __forceinline__ uint16_t bf16_bits_exact(double v) {
@@ -93,32 +93,58 @@ __device__ __forceinline__ uint16_t bf16_bits_exact(double v) {
   return static_cast<uint16_t>(__float_as_uint(f) >> 16);
 }
 
-// Balanced base-255 digits of y * 2^46 for |y| < 1 (slice 0 most significant).
-// All steps are exact in FP64: q is the integer nearest to Y / 255 (the
-// product's error stays far below the 1/510 distance of Y / 255 from a half
-// integer), and Y - 255 q is an integer of magnitude <= 127.
-__device__ __forceinline__ void i8_digits(double y, int8_t (&d)[S8]) {
-  double Y = rint(y * 70368744177664.0);   // 2^46
-#pragma unroll
-  for (int s = S8 - 1; s >= 0; --s) {
-    double q = rint(Y * (1.0 / 255.0));
-    double r = fma(-q, 255.0, Y);
-    if (r > 127.0) {
-      r -= 255.0;
-      q += 1.0;
-    } else if (r < -127.0) {
-      r += 255.0;
-      q -= 1.0;
-    }
-    d[s] = static_cast<int8_t>(static_cast<int>(r));
-    Y = q;
+// Source view of one pack item: A' (rows = saved, cols = contracted) or B'^T
+// (rows = saved = cols of B's descriptor, cols = contracted = rows of B's
+// descriptor).  A plain A block has row r, col p at r * cols + p (k
+// contiguous); a plain B block seen transposed has row j, col p at
+// p * rows + j (rows contiguous).
+struct PackView {
+  const double *src;
+  int64_t base;
+  int32_t rows, cols;
+  const int32_t *rdim, *rstr, *cdim, *cstr, *rsh, *csh;
+  const uint64_t *rmag, *cmag;
+  int nr, nc;
+  bool is_a, plain;
+  bool kfast;   // whether consecutive k are consecutive in memory
+  __device__ __forceinline__ int32_t row_off(int r) const {
+    return plain ? (is_a ? r * cols : r) : split_offset(r, rdim, rstr, rmag, rsh, nr);
   }
+  __device__ __forceinline__ int32_t col_off(int p) const {
+    return plain ? (is_a ? p : p * rows) : split_offset(p, cdim, cstr, cmag, csh, nc);
+  }
+};
+
+__device__ __forceinline__ PackView pack_view(bool is_a, const PackMeta &pm, const BlkGather &g, int64_t base,
+                                              const GatherLayout &L, const double *src) {
+  PackView v;
+  v.src = src;
+  v.base = base;
+  v.rows = pm.rows;
+  v.cols = pm.cols;
+  v.rdim = is_a ? g.rdim : g.cdim;
+  v.rstr = is_a ? g.rstr : g.cstr;
+  v.cdim = is_a ? g.cdim : g.rdim;
+  v.cstr = is_a ? g.cstr : g.rstr;
+  v.rmag = is_a ? g.rmag : g.cmag;
+  v.rsh = is_a ? g.rsh : g.csh;
+  v.cmag = is_a ? g.cmag : g.rmag;
+  v.csh = is_a ? g.csh : g.rsh;
+  v.nr = is_a ? L.a_nr : L.b_nc;
+  v.nc = is_a ? L.a_nc : L.b_nr;
+  v.is_a = is_a;
+  v.plain = g.plain != 0;
+  v.kfast = v.plain ? is_a : (is_a ? (g.fastcol != 0) : (g.fastcol == 0));
+  return v;
 }
 
-// Pack kernel: one workgroup of 256 threads per (side, block, 64-row group).
-// Pass 1 finds the max |x| of each row over all k; pass 2 splits and writes
-// BF16 slices or (I8) int8 digits.
-template <bool CPLX, bool I8>
+// Pack kernels, 256 threads per workgroup, in two launches.  Phase 1: one
+// workgroup per (side, block, 64-row group) finds the max |x| of each row
+// over all k and writes the row exponents.  Phase 2: one workgroup per
+// (side, block, 64-row group, k-chunk of KP) splits its chunk into BF16
+// slices or (I8) int8 digits and writes them, so the digit work of a long
+// block spreads over many workgroups.
+template <bool CPLX, bool I8, int PHASE>
 __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items,
                                               const PackMeta *__restrict__ meta_a,
                                               const PackMeta *__restrict__ meta_b,
@@ -128,95 +154,107 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
                                               const int64_t *__restrict__ b_offs, const GatherLayout L,
                                               const double *__restrict__ A, const double *__restrict__ B,
                                               void *__restrict__ planes_v, int32_t *__restrict__ exps) {
-  constexpr int KP = CPLX ? 32 : 64;   // k per pass-2 chunk
+  constexpr int KP = CPLX ? 32 : 64;   // k per phase-2 chunk
   constexpr int KCF = I8 ? KC8 : KC;   // k per packed chunk
   constexpr int PARTS = CPLX ? 2 : 1;
   __shared__ int32_t s_roff[64];
   __shared__ int32_t s_coff[64];
-  __shared__ unsigned long long s_max[64];
+  __shared__ double s_max[64];
+  __shared__ double s_max4[4][64];
   __shared__ double s_tile[PARTS][64][KP + 1];
 
   const PackItem it = items[blockIdx.x];
   const bool is_a = (it.side == 0);
   const PackMeta pm = is_a ? meta_a[it.blk] : meta_b[it.blk];
-  const BlkGather &g = is_a ? ga[it.blk] : gb[it.blk];
-  const int64_t base = is_a ? a_offs[it.blk] : b_offs[it.blk];
-  const double *src = is_a ? A : B;
-  // A': rows = saved (rdim), cols = contracted (cdim).  B'^T: rows = saved
-  // (cdim of B's descriptor), cols = contracted (rdim of B's descriptor).
-  const int32_t *rdim = is_a ? g.rdim : g.cdim;
-  const int32_t *rstr = is_a ? g.rstr : g.cstr;
-  const int32_t *cdim = is_a ? g.cdim : g.rdim;
-  const int32_t *cstr = is_a ? g.cstr : g.rstr;
-  const uint64_t *rmag = is_a ? g.rmag : g.cmag;
-  const int32_t *rsh = is_a ? g.rsh : g.csh;
-  const uint64_t *cmag = is_a ? g.cmag : g.rmag;
-  const int32_t *csh = is_a ? g.csh : g.rsh;
-  const int nr = is_a ? L.a_nr : L.b_nc;
-  const int nc = is_a ? L.a_nc : L.b_nr;
-  // A plain block: row r, col p at r * cols + p (k contiguous).  A plain B
-  // block seen transposed: row j, col p at p * rows + j (rows contiguous).
-  const bool plain = g.plain != 0;
-  // whether consecutive k are consecutive in memory
-  const bool kfast = plain ? is_a : (is_a ? (g.fastcol != 0) : (g.fastcol == 0));
+  const PackView v = pack_view(is_a, pm, is_a ? ga[it.blk] : gb[it.blk], is_a ? a_offs[it.blk] : b_offs[it.blk], L,
+                               is_a ? A : B);
+  const double *src = v.src;
+  const int64_t base = v.base;
+  const bool kfast = v.kfast;
   const int tid = threadIdx.x;
-  auto row_off = [&](int r) { return plain ? (is_a ? r * pm.cols : r) : split_offset(r, rdim, rstr, rmag, rsh, nr); };
-  auto col_off = [&](int p) { return plain ? (is_a ? p : p * pm.rows) : split_offset(p, cdim, cstr, cmag, csh, nc); };
+  auto row_off = [&](int r) { return v.row_off(r); };
+  auto col_off = [&](int p) { return v.col_off(p); };
 
   if (tid < 64) {
     const int r = it.r0 + tid;
     s_roff[tid] = (r < pm.rows) ? row_off(r) : -1;
-    s_max[tid] = 0ull;
   }
   __syncthreads();
 
+  __shared__ int32_t s_exp[64];
+  if constexpr (PHASE == 1) {
   // ---- pass 1: row maxima ----
+  // Each thread keeps running maxima of the elements it reads; no atomics.
+  // k contiguous: wave w reads 64 consecutive k of rows w, w + 4, ..., w + 60
+  // (one row per load, coalesced) and reduces each row's maximum across its
+  // lanes at the end.  Rows contiguous: lane l reads row l at k = w, w + 4,
+  // ... and the four waves' maxima meet in LDS.
+  auto absmax = [&](int64_t e) {
+    if (CPLX) {
+      const double2 v = reinterpret_cast<const double2 *>(src)[e];
+      return fmax(fabs(v.x), fabs(v.y));
+    }
+    return fabs(src[e]);
+  };
+  const int lane = tid & 63, wv = tid >> 6;
+  double mx[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) mx[q] = 0.0;
   for (int p0 = 0; p0 < pm.cols; p0 += 64) {
     if (tid < 64) {
       const int p = p0 + tid;
       s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
     }
     __syncthreads();
-    for (int q = 0; q < 16; ++q) {
-      int r, p;
-      if (kfast) {
-        p = tid & 63;
-        r = (tid >> 6) + 4 * q;
-      } else {
-        r = tid & 63;
-        p = (tid >> 6) + 4 * q;
+    if (kfast) {
+      const int32_t co = s_coff[lane];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int32_t ro = s_roff[wv + 4 * q];
+        if (ro >= 0 && co >= 0) mx[q] = fmax(mx[q], absmax(base + ro + co));
       }
-      const int32_t ro = s_roff[r], co = s_coff[p];
-      if (ro >= 0 && co >= 0) {
-        const int64_t e = base + ro + co;
-        double m;
-        if (CPLX) {
-          const double2 v = reinterpret_cast<const double2 *>(src)[e];
-          m = fmax(fabs(v.x), fabs(v.y));
-        } else {
-          m = fabs(src[e]);
-        }
-        atomicMax(&s_max[r], static_cast<unsigned long long>(__double_as_longlong(m)));
+    } else {
+      const int32_t ro = s_roff[lane];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int32_t co = s_coff[wv + 4 * q];
+        if (ro >= 0 && co >= 0) mx[0] = fmax(mx[0], absmax(base + ro + co));
       }
     }
     __syncthreads();
   }
+  if (kfast) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      double m = mx[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+      if (lane == 0) s_max[wv + 4 * q] = m;
+    }
+  } else {
+    s_max4[wv][lane] = mx[0];
+  }
+  __syncthreads();
+  if (!kfast && tid < 64)
+    s_max[tid] = fmax(fmax(s_max4[0][tid], s_max4[1][tid]), fmax(s_max4[2][tid], s_max4[3][tid]));
+  __syncthreads();
 
   // exponent of each row: max / 2^e in [1/2, 1]
-  __shared__ int32_t s_exp[64];
   if (tid < 64) {
-    const double m = __longlong_as_double(static_cast<long long>(s_max[tid]));
+    const double m = s_max[tid];
     int e = 0;
     if (m > 0.0) (void)frexp(m, &e);
-    s_exp[tid] = e;
     const int r = it.r0 + tid;
     if (r < pm.r_pad) exps[pm.exp_off + r] = e;
   }
-  __syncthreads();
+  return;
+  }
 
-  // ---- pass 2: split and write (including the zero padding) ----
+  // ---- phase 2: split and write one chunk (including the zero padding) ----
+  if (tid < 64) s_exp[tid] = exps[pm.exp_off + it.r0 + tid];   // r0 + 63 < r_pad
   const int nkc = pm.k_pad / KCF;
-  for (int p0 = 0; p0 < pm.k_pad; p0 += KP) {
+  {
+    const int p0 = it.k0;
     if (tid < KP) {
       const int p = p0 + tid;
       s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
@@ -250,36 +288,42 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
     const int grow = it.r0 + r;
     if (I8 && grow < pm.r_pad) {
       int8_t *planes = static_cast<int8_t *>(planes_v);
-      const double inv = ldexp(1.0, -s_exp[r]);
+      // x 2^(46 - e) is exact (a power of two) and below 2^46 in magnitude.
+      // Digits from the least significant up, all exact in FP64: q is the
+      // integer nearest to Y / 255 (the product's error, below 2^-14, stays
+      // far inside the 1/510 distance of Y / 255 from a half integer), so
+      // Y - 255 q is the balanced digit in [-127, 127].
+      const double sc = ldexp(1.0, 46 - s_exp[r]);
 #pragma unroll
       for (int part = 0; part < PARTS; ++part) {
-        int8_t out[S8][RUN];
+        uint32_t wd[S8][RUN / 4];
+#pragma unroll
+        for (int sl = 0; sl < S8; ++sl)
+#pragma unroll
+          for (int v = 0; v < RUN / 4; ++v) wd[sl][v] = 0u;
 #pragma unroll
         for (int u = 0; u < RUN; ++u) {
-          int8_t d[S8];
-          i8_digits(s_tile[part][r][pr + u] * inv, d);   // the scaling is exact (power of two)
+          double Y = rint(s_tile[part][r][pr + u] * sc);
 #pragma unroll
-          for (int s = 0; s < S8; ++s) out[s][u] = d[s];
+          for (int sl = S8 - 1; sl >= 0; --sl) {
+            const double q = rint(Y * (1.0 / 255.0));
+            const int d = static_cast<int>(fma(-q, 255.0, Y));
+            wd[sl][u >> 2] |= static_cast<uint32_t>(d & 0xff) << (8 * (u & 3));
+            Y = q;
+          }
         }
         const int p = p0 + pr;
         if (p < pm.k_pad) {
           // RUN (8 or 16) consecutive k never cross a KC8 boundary
 #pragma unroll
-          for (int s = 0; s < S8; ++s) {
+          for (int sl = 0; sl < S8; ++sl) {
             const int64_t idx =
                 pm.plane_off +
-                ((static_cast<int64_t>(part * S8 + s) * nkc + p / KC8) * pm.r_pad + grow) * KC8 + (p % KC8);
-            uint32_t w[RUN / 4];
-#pragma unroll
-            for (int u = 0; u < RUN / 4; ++u)
-              w[u] = static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u])) |
-                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 1])) << 8) |
-                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 2])) << 16) |
-                     (static_cast<uint32_t>(static_cast<uint8_t>(out[s][4 * u + 3])) << 24);
+                ((static_cast<int64_t>(part * S8 + sl) * nkc + p / KC8) * pm.r_pad + grow) * KC8 + (p % KC8);
             if constexpr (RUN == 16) {
-              *reinterpret_cast<v4u *>(planes + idx) = v4u{w[0], w[1], w[2], w[3]};
+              *reinterpret_cast<v4u *>(planes + idx) = v4u{wd[sl][0], wd[sl][1], wd[sl][2], wd[sl][3]};
             } else {
-              *reinterpret_cast<uint2 *>(planes + idx) = uint2{w[0], w[1]};
+              *reinterpret_cast<uint2 *>(planes + idx) = uint2{wd[sl][0], wd[sl][1]};
             }
           }
         }
@@ -327,10 +371,201 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   }
 }
 
+// Single-read pack of the blocks whose k_pad is at most kShortK: one
+// workgroup of 256 threads per (side, block, group of RW rows) stages the
+// rows' whole k extent in LDS (the block is read once), then the KW / 8
+// threads of a row take the row maximum and split 8 consecutive k each.  KW
+// (64, 128 or 256, at least k_pad; carried in the item's k0 field) and RW =
+// 2048 / KW keep every lane busy on short blocks.  LDS rows hold one pad
+// double per 8 so that the 8-element runs are read without bank conflicts.
+constexpr int kShortK = 256;
+template <bool CPLX, bool I8>
+__global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__ items,
+                                                    const PackMeta *__restrict__ meta_a,
+                                                    const PackMeta *__restrict__ meta_b,
+                                                    const BlkGather *__restrict__ ga,
+                                                    const BlkGather *__restrict__ gb,
+                                                    const int64_t *__restrict__ a_offs,
+                                                    const int64_t *__restrict__ b_offs, const GatherLayout L,
+                                                    const double *__restrict__ A, const double *__restrict__ B,
+                                                    void *__restrict__ planes_v, int32_t *__restrict__ exps) {
+  constexpr int PARTS = CPLX ? 2 : 1;
+  constexpr int RUN = 8;                         // consecutive k per thread
+  constexpr int ELEMS = 256 * RUN;               // elements per part per workgroup
+  constexpr int KCF = I8 ? KC8 : KC;
+  __shared__ int32_t s_roff[ELEMS / 64];
+  __shared__ int32_t s_coff[kShortK];
+  __shared__ double s_tile[PARTS][ELEMS + ELEMS / 8 + 32];
+
+  const PackItem it = items[blockIdx.x];
+  const int KW = it.k0, RW = ELEMS / KW, TPR = KW / RUN, lrow = KW + KW / 8 + 1;
+  const bool is_a = (it.side == 0);
+  const PackMeta pm = is_a ? meta_a[it.blk] : meta_b[it.blk];
+  const PackView v = pack_view(is_a, pm, is_a ? ga[it.blk] : gb[it.blk], is_a ? a_offs[it.blk] : b_offs[it.blk], L,
+                               is_a ? A : B);
+  const int tid = threadIdx.x;
+  if (tid < RW) {
+    const int r = it.r0 + tid;
+    s_roff[tid] = (r < pm.rows) ? v.row_off(r) : -1;
+  }
+  if (tid < KW) s_coff[tid] = (tid < pm.cols) ? v.col_off(tid) : -1;
+  __syncthreads();
+
+  // ---- one read of the RW rows (zeros outside the block) ----
+#pragma unroll
+  for (int j = 0; j < RUN; ++j) {
+    int r, p;
+    if (v.kfast) {
+      const int f = tid + 256 * j;               // KW consecutive k per row
+      r = f / KW;
+      p = f % KW;
+    } else {
+      const int f = tid + 256 * j;               // RW consecutive rows per k
+      r = f % RW;
+      p = f / RW;
+    }
+    const int32_t ro = s_roff[r], co = s_coff[p];
+    const bool ok = (ro >= 0) && (co >= 0);
+    const int64_t e = v.base + ro + co;
+    const int lp = r * lrow + p + (p >> 3);
+    if (CPLX) {
+      const double2 x = ok ? reinterpret_cast<const double2 *>(v.src)[e] : double2{0.0, 0.0};
+      s_tile[0][lp] = x.x;
+      s_tile[PARTS - 1][lp] = x.y;
+    } else {
+      s_tile[0][lp] = ok ? v.src[e] : 0.0;
+    }
+  }
+  __syncthreads();
+
+  // ---- row maximum and exponent (TPR threads per row) ----
+  const int r = tid / TPR, run = tid % TPR, p0 = run * RUN;
+  double x[PARTS][RUN];
+  double m = 0.0;
+#pragma unroll
+  for (int part = 0; part < PARTS; ++part)
+#pragma unroll
+    for (int u = 0; u < RUN; ++u) {
+      x[part][u] = s_tile[part][r * lrow + 9 * run + u];
+      m = fmax(m, fabs(x[part][u]));
+    }
+  for (int o = TPR / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, TPR));
+  int ex = 0;
+  if (m > 0.0) (void)frexp(m, &ex);
+  const int grow = it.r0 + r;                    // < r_pad
+  if (run == 0) exps[pm.exp_off + grow] = ex;
+  if (p0 >= pm.k_pad) return;
+
+  // ---- split the run p0 .. p0 + 7 (inside one packed k-chunk) ----
+  const int nkc = pm.k_pad / KCF;
+  if constexpr (I8) {
+    int8_t *planes = static_cast<int8_t *>(planes_v);
+    const double sc = ldexp(1.0, 46 - ex);
+#pragma unroll
+    for (int part = 0; part < PARTS; ++part) {
+      uint32_t wd[S8][2] = {};
+#pragma unroll
+      for (int u = 0; u < RUN; ++u) {
+        double Y = rint(x[part][u] * sc);
+#pragma unroll
+        for (int sl = S8 - 1; sl >= 0; --sl) {
+          const double q = rint(Y * (1.0 / 255.0));
+          const int d = static_cast<int>(fma(-q, 255.0, Y));
+          wd[sl][u >> 2] |= static_cast<uint32_t>(d & 0xff) << (8 * (u & 3));
+          Y = q;
+        }
+      }
+#pragma unroll
+      for (int sl = 0; sl < S8; ++sl) {
+        const int64_t idx =
+            pm.plane_off + ((static_cast<int64_t>(part * S8 + sl) * nkc + p0 / KC8) * pm.r_pad + grow) * KC8 + (p0 % KC8);
+        *reinterpret_cast<uint2 *>(planes + idx) = uint2{wd[sl][0], wd[sl][1]};
+      }
+    }
+  } else {
+    uint16_t *planes = static_cast<uint16_t *>(planes_v);
+    const double inv = ldexp(1.0, -ex);
+#pragma unroll
+    for (int part = 0; part < PARTS; ++part) {
+      uint16_t out[S][RUN];
+#pragma unroll
+      for (int u = 0; u < RUN; ++u) {
+        double y = x[part][u] * inv;                 // exact (power of two)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const double scale = ldexp(1.0, 8 + 9 * s);
+          const double c = rint(y * scale);           // |c| <= 256
+          y -= c / scale;                             // exact
+          out[s][u] = bf16_bits_exact(c * ldexp(1.0, -9 * s));
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int64_t idx =
+            pm.plane_off + ((static_cast<int64_t>(part * S + s) * nkc + p0 / KC) * pm.r_pad + grow) * KC + (p0 % KC);
+        v4u w;
+        w[0] = static_cast<uint32_t>(out[s][0]) | (static_cast<uint32_t>(out[s][1]) << 16);
+        w[1] = static_cast<uint32_t>(out[s][2]) | (static_cast<uint32_t>(out[s][3]) << 16);
+        w[2] = static_cast<uint32_t>(out[s][4]) | (static_cast<uint32_t>(out[s][5]) << 16);
+        w[3] = static_cast<uint32_t>(out[s][6]) | (static_cast<uint32_t>(out[s][7]) << 16);
+        *reinterpret_cast<v4u *>(planes + idx) = w;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ v8bf neg8(v8bf x) {
   v4u u = __builtin_bit_cast(v4u, x);
   u ^= 0x80008000u;
   return __builtin_bit_cast(v8bf, u);
+}
+
+// Self-contained tile and task records of the persistent split kernels, so
+// that a tile or task boundary reads one record instead of a chain of
+// tile -> segment -> task -> block metadata.
+struct SplitTile {
+  int64_t c_off;    // output block
+  int32_t m, n;
+  int32_t i0, j0;   // tile origin in the output block
+  int32_t t0, t1;   // the block's tasks
+};
+struct SplitTask {
+  int64_t a_plane, b_plane;   // plane offsets of the task's A and B blocks
+  int64_t a_exp, b_exp;       // exponent offsets
+  int32_t a_rpad, b_rpad;     // padded rows
+  int32_t nkc;                // packed k-chunks
+  int32_t pad;
+};
+
+// Record loads whose fields are pinned to scalar registers (the records are
+// uniform across the workgroup).
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ SplitTile load_tile(const SplitTile *p, int i) {
+  SplitTile r = p[i];
+  r.c_off = uniform64(r.c_off);
+  r.m = __builtin_amdgcn_readfirstlane(r.m);
+  r.n = __builtin_amdgcn_readfirstlane(r.n);
+  r.i0 = __builtin_amdgcn_readfirstlane(r.i0);
+  r.j0 = __builtin_amdgcn_readfirstlane(r.j0);
+  r.t0 = __builtin_amdgcn_readfirstlane(r.t0);
+  r.t1 = __builtin_amdgcn_readfirstlane(r.t1);
+  return r;
+}
+__device__ __forceinline__ SplitTask load_task(const SplitTask *p, int i) {
+  SplitTask r = p[i];
+  r.a_plane = uniform64(r.a_plane);
+  r.b_plane = uniform64(r.b_plane);
+  r.a_exp = uniform64(r.a_exp);
+  r.b_exp = uniform64(r.b_exp);
+  r.a_rpad = __builtin_amdgcn_readfirstlane(r.a_rpad);
+  r.b_rpad = __builtin_amdgcn_readfirstlane(r.b_rpad);
+  r.nkc = __builtin_amdgcn_readfirstlane(r.nkc);
+  return r;
 }
 
 // Tile order for the XCDs: workgroup b runs on XCD b % 8.  Mapping b to
@@ -933,8 +1168,7 @@ __global__ __launch_bounds__(256) void k_split_i8_f64(
 // crosses task and tile boundaries; the imaginary parts of a finished tile go
 // to the real-part waves through an LDS exchange area) on int8 digits.
 __global__ __launch_bounds__(512) void k_split_i8_c2(
-    const Tile *__restrict__ tiles, int32_t n_tiles, const Seg *__restrict__ segs,
-    const ExecTask *__restrict__ exec, const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
+    const SplitTile *__restrict__ stiles, int32_t n_tiles, const SplitTask *__restrict__ stasks,
     const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int PARTS = 2;
@@ -953,19 +1187,6 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
   const int lrow = lane & 31, lh = lane >> 5;
   const int G = static_cast<int>(gridDim.x);
 
-  struct TileRef {
-    int i0, j0;
-    Seg seg;
-  };
-  auto tile_at = [&](int bidx) __attribute__((always_inline)) {
-    TileRef r;
-    const Tile tl = tiles[split_xcd_tile(bidx, n_tiles)];
-    r.seg = segs[tl.seg];
-    r.i0 = tl.tm * TM;
-    r.j0 = tl.tn * TN;
-    return r;
-  };
-
   double acc64[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
@@ -978,19 +1199,18 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
   // byte offsets of a thread's pieces within a task's k-chunk 0 (a block's
   // digit planes stay below 2^31 bytes), recomputed when the task changes
   uint32_t oa[PER_THREAD], ob[PER_THREAD];
-  auto task_offsets = [&](const PackMeta &ma, const PackMeta &mb) __attribute__((always_inline)) {
-    const int nkc = ma.k_pad / KC8;
+  auto task_offsets = [&](const SplitTask &tk) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < PER_THREAD; ++u) {
       const int c = tid + NT * u;
       const int ps = c >> 7, rr = (c & 127) >> 1, half = c & 1;
-      oa[u] = static_cast<uint32_t>((ps * nkc * ma.r_pad + rr) * KC8 + half * 16);
-      ob[u] = static_cast<uint32_t>((ps * nkc * mb.r_pad + rr) * KC8 + half * 16);
+      oa[u] = static_cast<uint32_t>((ps * tk.nkc * tk.a_rpad + rr) * KC8 + half * 16);
+      ob[u] = static_cast<uint32_t>((ps * tk.nkc * tk.b_rpad + rr) * KC8 + half * 16);
     }
   };
-  auto fetch = [&](const PackMeta &ma, const PackMeta &mb, int kc, int i0, int j0) __attribute__((always_inline)) {
-    const int8_t *ba = planes + ma.plane_off + (static_cast<int64_t>(kc) * ma.r_pad + i0) * KC8;
-    const int8_t *bb = planes + mb.plane_off + (static_cast<int64_t>(kc) * mb.r_pad + j0) * KC8;
+  auto fetch = [&](const SplitTask &tk, int kc, int i0, int j0) __attribute__((always_inline)) {
+    const int8_t *ba = planes + tk.a_plane + (static_cast<int64_t>(kc) * tk.a_rpad + i0) * KC8;
+    const int8_t *bb = planes + tk.b_plane + (static_cast<int64_t>(kc) * tk.b_rpad + j0) * KC8;
 #pragma unroll
     for (int u = 0; u < PER_THREAD; ++u) {
       pa[u] = *reinterpret_cast<const v4u *>(ba + oa[u]);
@@ -1007,50 +1227,44 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
     }
   };
 
+  // The tile and task records are read one step ahead of their use: `nxt`
+  // is the workgroup's next tile, `tk_n` the task after the current one (in
+  // this tile or the next), so a boundary waits only on plane loads.
   int bt = static_cast<int>(blockIdx.x);
-  TileRef cur = tile_at(bt);
-  int t = cur.seg.t0, kc = 0;
-  PackMeta ma = meta_a[exec[t].a_blk], mb = meta_b[exec[t].b_blk];
-  int nkc = ma.k_pad / KC8;
-  int32_t eb = exps[mb.exp_off + cur.j0 + wn + lrow];
-  task_offsets(ma, mb);
-  fetch(ma, mb, 0, cur.i0, cur.j0);
-  if (tid < TM) s_ea[0][tid] = exps[ma.exp_off + cur.i0 + tid];
+  SplitTile cur = load_tile(stiles, split_xcd_tile(bt, n_tiles));
+  const bool more0 = bt + G < n_tiles;
+  SplitTile nxt = more0 ? load_tile(stiles, split_xcd_tile(bt + G, n_tiles)) : cur;
+  int t = cur.t0, kc = 0;
+  SplitTask tk = load_task(stasks, t);
+  SplitTask tk_n = (t + 1 < cur.t1) ? load_task(stasks, t + 1) : (more0 ? load_task(stasks, nxt.t0) : tk);
+  int32_t eb = exps[tk.b_exp + cur.j0 + wn + lrow];
+  task_offsets(tk);
+  fetch(tk, 0, cur.i0, cur.j0);
+  if (tid < TM) s_ea[0][tid] = exps[tk.a_exp + cur.i0 + tid];
   stage(0);
   __syncthreads();
   int buf = 0;
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 
   while (true) {
-    int t2 = t, kc2 = kc + 1;
-    bool new_task = false;
-    if (kc2 == nkc) {
-      t2 = t + 1;
-      kc2 = 0;
-      new_task = true;
-    }
-    const bool tile_end = t2 >= cur.seg.t1;
+    const bool new_task = kc + 1 == tk.nkc;
+    const bool tile_end = new_task && t + 1 >= cur.t1;
     const int bt2 = tile_end ? bt + G : bt;
     const bool has_next = bt2 < n_tiles;
-    TileRef nxt = cur;
-    PackMeta ma2 = ma, mb2 = mb;
+    const int i0n = tile_end ? nxt.i0 : cur.i0, j0n = tile_end ? nxt.j0 : cur.j0;
+    const int kc2 = new_task ? 0 : kc + 1;
     int32_t eb2 = eb;
     if (has_next) {
-      if (tile_end) {
-        nxt = tile_at(bt2);
-        t2 = nxt.seg.t0;
-      }
       if (new_task) {
-        const ExecTask e2 = exec[t2];
-        ma2 = meta_a[e2.a_blk];
-        mb2 = meta_b[e2.b_blk];
-        eb2 = exps[mb2.exp_off + nxt.j0 + wn + lrow];
-        if (tid < TM) ea_next = exps[ma2.exp_off + nxt.i0 + tid];
-        task_offsets(ma2, mb2);
+        eb2 = exps[tk_n.b_exp + j0n + wn + lrow];
+        if (tid < TM) ea_next = exps[tk_n.a_exp + i0n + tid];
+        task_offsets(tk_n);
+        fetch(tk_n, 0, i0n, j0n);
+      } else {
+        fetch(tk, kc2, i0n, j0n);
       }
-      fetch(ma2, mb2, kc2, nxt.i0, nxt.j0);
     }
-    const bool active = (cur.i0 + wm < cur.seg.m) && (cur.j0 + wn < cur.seg.n);
+    const bool active = (cur.i0 + wm < cur.m) && (cur.j0 + wn < cur.n);
     if (active) {
       const int brow = wn + lrow, arow = wm + lrow;
       v4i b0[S8], b1[S8];
@@ -1071,7 +1285,7 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
         }
       }
       const int kk = (kc + 1) * KC8;                // k since the task start
-      if (kc == nkc - 1 || kk % kI8FlushK == 0) {
+      if (new_task || kk % kI8FlushK == 0) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int row = wm + 8 * (r >> 2) + 4 * lh + (r & 3);
@@ -1092,8 +1306,8 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
-          if (row < cur.seg.m && col < cur.seg.n) {
-            const int64_t idx = cur.seg.c_off + static_cast<int64_t>(row) * cur.seg.n + col;
+          if (row < cur.m && col < cur.n) {
+            const int64_t idx = cur.c_off + static_cast<int64_t>(row) * cur.n + col;
             const double xr = acc64[r], xi = xch[ws][r][lane];
             double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
             double2 *cp = reinterpret_cast<double2 *>(C) + idx;
@@ -1115,17 +1329,22 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
     }
     __syncthreads();
     if (!has_next) break;
-    if (tile_end) {
-      cur = nxt;
-      bt = bt2;
-    }
+    // ---- advance, and read the records one step ahead ----
     if (new_task) {
-      ma = ma2;
-      mb = mb2;
-      nkc = ma.k_pad / KC8;
+      t = tile_end ? nxt.t0 : t + 1;
+      tk = tk_n;
       eb = eb2;
+      if (tile_end) {
+        bt = bt2;
+        cur = nxt;
+        if (bt + G < n_tiles) nxt = load_tile(stiles, split_xcd_tile(bt + G, n_tiles));
+      }
+      const bool more = bt + G < n_tiles;
+      if (t + 1 < cur.t1)
+        tk_n = load_task(stasks, t + 1);
+      else if (more)
+        tk_n = load_task(stasks, nxt.t0);
     }
-    t = t2;
     kc = kc2;
     buf ^= 1;
   }
@@ -1143,18 +1362,23 @@ inline SplitFmt split_fmt(int32_t mode) { return mode == QLT_MODE_SPLIT_I8 ? Spl
 // Host-side layout of the split workspace of a plan.
 struct SplitLayout {
   std::vector<PackMeta> meta_a, meta_b;
-  std::vector<PackItem> items;
+  std::vector<PackItem> items;    // phase 1: (side, block, 64-row group)
+  std::vector<PackItem> citems;   // phase 2: (side, block, 64-row group, k-chunk)
+  std::vector<PackItem> sitems;   // single-read pack of k_pad <= kShortK: (side, block, row group, k width)
+  std::vector<SplitTile> rtiles;  // self-contained records of `tiles`
+  std::vector<SplitTask> rtasks;  // self-contained records of the plan's tasks
   std::vector<Tile> tiles;        // 64 x 64 output tiles of the MFMA-class blocks, longest first
   int64_t plane_elems = 0;
   int64_t exp_elems = 0;
   size_t off_meta_a = 0, off_meta_b = 0, off_items = 0, off_aoffs = 0, off_boffs = 0, off_exps = 0,
-         off_planes = 0, off_tiles = 0, total = 0;
+         off_planes = 0, off_tiles = 0, off_citems = 0, off_sitems = 0, off_rtiles = 0, off_rtasks = 0, total = 0;
 };
 
 SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
   SplitLayout L;
   const int parts = (dtype == QLT_C128) ? 2 : 1;
   const SplitFmt fmt = split_fmt(mode);
+  const int32_t kp = (dtype == QLT_C128) ? 32 : 64;   // k per phase-2 pack chunk
   L.meta_a.assign(plan->n_a_blks, PackMeta{0, 0, 0, 0, 0, 0});
   L.meta_b.assign(plan->n_b_blks, PackMeta{0, 0, 0, 0, 0, 0});
   std::vector<char> used_a(plan->n_a_blks, 0), used_b(plan->n_b_blks, 0);
@@ -1189,8 +1413,16 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
       m.exp_off = L.exp_elems;
       L.plane_elems += static_cast<int64_t>(parts) * fmt.slices * m.r_pad * m.k_pad;
       L.exp_elems += m.r_pad;
-      for (int32_t r0 = 0; r0 < m.r_pad; r0 += 64)
+      if (m.k_pad <= kShortK) {
+        const int32_t kw = m.k_pad <= 64 ? 64 : (m.k_pad <= 128 ? 128 : 256), rw = 2048 / kw;
+        for (int32_t r0 = 0; r0 < m.r_pad; r0 += rw) L.sitems.push_back(PackItem{side, static_cast<int32_t>(i), r0, kw});
+        continue;
+      }
+      for (int32_t r0 = 0; r0 < m.r_pad; r0 += 64) {
         L.items.push_back(PackItem{side, static_cast<int32_t>(i), r0, 0});
+        for (int32_t k0 = 0; k0 < m.k_pad; k0 += kp)
+          L.citems.push_back(PackItem{side, static_cast<int32_t>(i), r0, k0});
+      }
     }
   };
   assign(L.meta_a, used_a, 0);
@@ -1210,6 +1442,18 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
         for (int b = 0; b < ntn; ++b) L.tiles.push_back(Tile{e.second, a, b, 0, 0, 0, 0, -1});
     }
   }
+  for (const Tile &tl : L.tiles) {
+    const Seg &sg = plan->h_segs[tl.seg];
+    L.rtiles.push_back(SplitTile{sg.c_off, sg.m, sg.n, tl.tm * TM, tl.tn * TN, sg.t0, sg.t1});
+  }
+  L.rtasks.assign(plan->h_exec.size(), SplitTask{0, 0, 0, 0, 0, 0, 0, 0});
+  for (size_t t = 0; t < plan->h_exec.size(); ++t) {
+    const ExecTask &e = plan->h_exec[t];
+    const PackMeta &ma = L.meta_a[e.a_blk], &mb = L.meta_b[e.b_blk];
+    if (ma.k_pad == 0 || mb.k_pad == 0) continue;   // a row-class task: never split
+    L.rtasks[t] = SplitTask{ma.plane_off, mb.plane_off, ma.exp_off, mb.exp_off, ma.r_pad, mb.r_pad,
+                            ma.k_pad / fmt.kc, 0};
+  }
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };
   size_t off = 0;
   L.off_meta_a = off;
@@ -1228,6 +1472,14 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
   off = align(off + static_cast<size_t>(fmt.elem_bytes) * L.plane_elems);
   L.off_tiles = off;
   off = align(off + sizeof(Tile) * L.tiles.size());
+  L.off_citems = off;
+  off = align(off + sizeof(PackItem) * L.citems.size());
+  L.off_sitems = off;
+  off = align(off + sizeof(PackItem) * L.sitems.size());
+  L.off_rtiles = off;
+  off = align(off + sizeof(SplitTile) * L.rtiles.size());
+  L.off_rtasks = off;
+  off = align(off + sizeof(SplitTask) * L.rtasks.size());
   L.total = off;
   return L;
 }
@@ -1320,12 +1572,22 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype, int32_t mode) {
   QLT_HIP(hipMemcpy(base + L.off_aoffs, aoffs.data(), sizeof(int64_t) * aoffs.size(), hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_boffs, boffs.data(), sizeof(int64_t) * boffs.size(), hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_tiles, L.tiles.data(), sizeof(Tile) * L.tiles.size(), hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_citems, L.citems.data(), sizeof(PackItem) * L.citems.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_sitems, L.sitems.data(), sizeof(PackItem) * L.sitems.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_rtiles, L.rtiles.data(), sizeof(SplitTile) * L.rtiles.size(),
+                    hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(base + L.off_rtasks, L.rtasks.data(), sizeof(SplitTask) * L.rtasks.size(),
+                    hipMemcpyHostToDevice));
   plan->split_dtype = dtype;
   plan->split_mode = mode;
   plan->split_items = static_cast<int64_t>(L.items.size());
   plan->split_tiles = static_cast<int64_t>(L.tiles.size());
-  const size_t offs[8] = {L.off_meta_a, L.off_meta_b, L.off_items, L.off_aoffs, L.off_boffs, L.off_exps,
-                          L.off_planes, L.off_tiles};
+  plan->split_citems = static_cast<int64_t>(L.citems.size());
+  plan->split_sitems = static_cast<int64_t>(L.sitems.size());
+  const size_t offs[12] = {L.off_meta_a, L.off_meta_b, L.off_items,  L.off_aoffs,  L.off_boffs,  L.off_exps,
+                           L.off_planes, L.off_tiles,  L.off_citems, L.off_sitems, L.off_rtiles, L.off_rtasks};
   std::memcpy(plan->split_off, offs, sizeof(offs));
   return QLT_OK;
 }
@@ -1347,6 +1609,12 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const
   auto *exps = reinterpret_cast<int32_t *>(base + so[5]);
   void *planes = base + so[6];
   const auto *stiles = reinterpret_cast<const Tile *>(base + so[7]);
+  const auto *citems = reinterpret_cast<const PackItem *>(base + so[8]);
+  const unsigned n_citems = static_cast<unsigned>(plan->split_citems);
+  const auto *sitems = reinterpret_cast<const PackItem *>(base + so[9]);
+  const auto *rtiles = reinterpret_cast<const SplitTile *>(base + so[10]);
+  const auto *rtasks = reinterpret_cast<const SplitTask *>(base + so[11]);
+  const unsigned n_sitems = static_cast<unsigned>(plan->split_sitems);
   const unsigned n_items = static_cast<unsigned>(plan->split_items);
   const unsigned n_mfma = static_cast<unsigned>(plan->split_tiles);
   const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
@@ -1366,32 +1634,40 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const
   auto *cd = static_cast<double *>(c);
   const auto *ad = static_cast<const double *>(a);
   const auto *bd = static_cast<const double *>(b);
+  // blocks with k_pad <= kShortK: single-read pack; longer ones: row exponents, then digits per k-chunk
+  auto launch_pack = [&](auto k_short, auto k_exp, auto k_dig) {
+    if (n_sitems > 0)
+      hipLaunchKernelGGL(k_short, dim3(n_sitems), blk, 0, st, sitems, meta_a, meta_b, plan->d_ga, plan->d_gb, aoffs,
+                         boffs, plan->layout, ad, bd, planes, exps);
+    if (n_items > 0) {
+      hipLaunchKernelGGL(k_exp, dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga, plan->d_gb, aoffs,
+                         boffs, plan->layout, ad, bd, planes, exps);
+      hipLaunchKernelGGL(k_dig, dim3(n_citems), blk, 0, st, citems, meta_a, meta_b, plan->d_ga, plan->d_gb, aoffs,
+                         boffs, plan->layout, ad, bd, planes, exps);
+    }
+  };
   // the complex kernels are persistent: one workgroup per CU (their LDS allows one)
   const unsigned grid = std::min<unsigned>(n_mfma, static_cast<unsigned>(device_cu_count()));
   if (!cplx && !i8) {
-    hipLaunchKernelGGL((k_pack<false, false>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
-                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    launch_pack(k_pack_short<false, false>, k_pack<false, false, 1>, k_pack<false, false, 2>);
     if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_gemm_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
                        meta_b, planes16, exps, cd, ar, br);
   } else if (cplx && !i8) {
-    hipLaunchKernelGGL((k_pack<true, false>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
-                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    launch_pack(k_pack_short<true, false>, k_pack<true, false, 1>, k_pack<true, false, 2>);
     if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
                        plan->d_segs, plan->d_exec, meta_a, meta_b, planes16, exps, cd, ar, ai, br, bi);
   } else if (!cplx) {
-    hipLaunchKernelGGL((k_pack<false, true>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
-                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    launch_pack(k_pack_short<false, true>, k_pack<false, true, 1>, k_pack<false, true, 2>);
     if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_i8_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
                        meta_b, planes8, exps, cd, ar, br);
   } else {
-    hipLaunchKernelGGL((k_pack<true, true>), dim3(n_items), blk, 0, st, items, meta_a, meta_b, plan->d_ga,
-                       plan->d_gb, aoffs, boffs, plan->layout, ad, bd, planes, exps);
+    launch_pack(k_pack_short<true, true>, k_pack<true, true, 1>, k_pack<true, true, 2>);
     if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_i8_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
-                       plan->d_segs, plan->d_exec, meta_a, meta_b, planes8, exps, cd, ar, ai, br, bi);
+    hipLaunchKernelGGL(k_split_i8_c2, dim3(grid), dim3(512), 0, st, rtiles, static_cast<int32_t>(n_mfma), rtasks,
+                       planes8, exps, cd, ar, ai, br, bi);
   }
   if (probe_split) {
     double elems = 0.0;

@@ -304,8 +304,10 @@ struct qlt_plan {
   int32_t split_dtype = -1;             // dtype the split metadata was written for
   int32_t split_mode = -1;              // split mode (digit format) it was written for
   int64_t split_items = 0;              // pack work items
-  size_t split_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // workspace sub-buffer offsets
+  size_t split_off[12] = {};            // workspace sub-buffer offsets
   int64_t split_tiles = 0;              // 64 x 64 tiles of the split kernel
+  int64_t split_citems = 0;             // phase-2 pack work items (row group x k-chunk)
+  int64_t split_sitems = 0;             // single-read pack work items (8-row groups of short blocks)
 
   qlt_plan() = default;
   qlt_plan(const qlt_plan &) = delete;
