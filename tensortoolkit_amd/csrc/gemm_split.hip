@@ -531,7 +531,7 @@ struct SplitTile {
 };
 struct SplitTask {
   int64_t a_plane, b_plane;   // plane offsets of the task's A and B blocks
-  int64_t a_exp, b_exp;       // exponent offsets
+  int32_t a_exp, b_exp;       // exponent offsets (a plan's exponents stay below 2^31)
   int32_t a_rpad, b_rpad;     // padded rows
   int32_t nkc;                // packed k-chunks
   int32_t pad;
@@ -560,8 +560,8 @@ __device__ __forceinline__ SplitTask load_task(const SplitTask *p, int i) {
   SplitTask r = p[i];
   r.a_plane = uniform64(r.a_plane);
   r.b_plane = uniform64(r.b_plane);
-  r.a_exp = uniform64(r.a_exp);
-  r.b_exp = uniform64(r.b_exp);
+  r.a_exp = __builtin_amdgcn_readfirstlane(r.a_exp);
+  r.b_exp = __builtin_amdgcn_readfirstlane(r.b_exp);
   r.a_rpad = __builtin_amdgcn_readfirstlane(r.a_rpad);
   r.b_rpad = __builtin_amdgcn_readfirstlane(r.b_rpad);
   r.nkc = __builtin_amdgcn_readfirstlane(r.nkc);
@@ -913,7 +913,11 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     // helps to stage operands (its MFMAs and flushes are skipped)
     const bool active = (cur.i0 + wm < cur.seg.m) && (cur.j0 + wn < cur.seg.n);
     if (active) {
-      // real part: ar*br + (-ai)*bi; imaginary part: ar*bi + ai*br
+      // real part: ar*br + (-ai)*bi; imaginary part: ar*bi + ai*br.  Slice 0
+      // of A is read first, then B slice by slice: the first MFMAs wait for
+      // three reads, and the later reads land while the earlier MFMAs run.
+      v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][0][0][wm + lrow][lk]);
+      v8bf ai = *reinterpret_cast<const v8bf *>(&As[buf][1][0][wm + lrow][lk]);
       v8bf b0[S], b1[S];
 #pragma unroll
       for (int s2 = 0; s2 < S; ++s2) {
@@ -922,8 +926,10 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
       }
 #pragma unroll
       for (int sa = 0; sa < S; ++sa) {
-        const v8bf ar = *reinterpret_cast<const v8bf *>(&As[buf][0][sa][wm + lrow][lk]);
-        v8bf ai = *reinterpret_cast<const v8bf *>(&As[buf][1][sa][wm + lrow][lk]);
+        if (sa > 0) {
+          ar = *reinterpret_cast<const v8bf *>(&As[buf][0][sa][wm + lrow][lk]);
+          ai = *reinterpret_cast<const v8bf *>(&As[buf][1][sa][wm + lrow][lk]);
+        }
         if (comp == 0) ai = neg8(ai);
 #pragma unroll
         for (int sb = 0; sb < S - sa; ++sb) {
@@ -1245,6 +1251,7 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
   __syncthreads();
   int buf = 0;
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  const bool real_scalars = (alpha_i == 0.0) && (beta_i == 0.0);
 
   while (true) {
     const bool new_task = kc + 1 == tk.nkc;
@@ -1267,16 +1274,24 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
     const bool active = (cur.i0 + wm < cur.m) && (cur.j0 + wn < cur.n);
     if (active) {
       const int brow = wn + lrow, arow = wm + lrow;
+      // real part ar*br + (-ai)*bi, imaginary part ar*bi + ai*br.  Digit 0
+      // of A first, then B digit by digit: the first MFMAs wait for three
+      // reads, and the later reads land while the earlier MFMAs run.
+      const int p0 = comp, p1 = 1 - comp;
+      v4i ar = *reinterpret_cast<const v4i *>(&As[buf][0][0][arow][i8_half(arow, lh)]);
+      v4i ai = *reinterpret_cast<const v4i *>(&As[buf][1][0][arow][i8_half(arow, lh)]);
       v4i b0[S8], b1[S8];
 #pragma unroll
       for (int s2 = 0; s2 < S8; ++s2) {
-        b0[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][comp][s2][brow][i8_half(brow, lh)]);
-        b1[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][1 - comp][s2][brow][i8_half(brow, lh)]);
+        b0[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][p0][s2][brow][i8_half(brow, lh)]);
+        b1[s2] = *reinterpret_cast<const v4i *>(&Bs[buf][p1][s2][brow][i8_half(brow, lh)]);
       }
 #pragma unroll
       for (int sa = 0; sa < S8; ++sa) {
-        const v4i ar = *reinterpret_cast<const v4i *>(&As[buf][0][sa][arow][i8_half(arow, lh)]);
-        v4i ai = *reinterpret_cast<const v4i *>(&As[buf][1][sa][arow][i8_half(arow, lh)]);
+        if (sa > 0) {
+          ar = *reinterpret_cast<const v4i *>(&As[buf][0][sa][arow][i8_half(arow, lh)]);
+          ai = *reinterpret_cast<const v4i *>(&As[buf][1][sa][arow][i8_half(arow, lh)]);
+        }
         if (comp == 0) ai = neg_digits(ai);
 #pragma unroll
         for (int sb = 0; sb < S8 - sa; ++sb) {
@@ -1295,7 +1310,24 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
         for (int l = 0; l < S8; ++l) g[l] = v16i{};
       }
     }
-    if (tile_end) {
+    if (tile_end && real_scalars) {
+      // each wave stores its own component (alpha and beta real)
+      if (active) {
+        const int col = cur.j0 + wn + lrow;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
+          if (row < cur.m && col < cur.n) {
+            double *cp = C + 2 * (cur.c_off + static_cast<int64_t>(row) * cur.n + col) + comp;
+            double y = alpha_r * acc64[r];
+            if (use_beta) y += beta_r * *cp;
+            *cp = y;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+    } else if (tile_end) {
       if (comp == 1) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) xch[ws][r][lane] = acc64[r];
@@ -1451,8 +1483,8 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
     const ExecTask &e = plan->h_exec[t];
     const PackMeta &ma = L.meta_a[e.a_blk], &mb = L.meta_b[e.b_blk];
     if (ma.k_pad == 0 || mb.k_pad == 0) continue;   // a row-class task: never split
-    L.rtasks[t] = SplitTask{ma.plane_off, mb.plane_off, ma.exp_off, mb.exp_off, ma.r_pad, mb.r_pad,
-                            ma.k_pad / fmt.kc, 0};
+    L.rtasks[t] = SplitTask{ma.plane_off, mb.plane_off, static_cast<int32_t>(ma.exp_off),
+                            static_cast<int32_t>(mb.exp_off), ma.r_pad, mb.r_pad, ma.k_pad / fmt.kc, 0};
   }
   auto align = [](size_t x) { return (x + 255) / 256 * 256; };
   size_t off = 0;
@@ -1557,6 +1589,8 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype, int32_t mode) {
   if (!split_profitable(plan, dtype, mode)) return QLT_OK;
   SplitLayout L = compute_layout(plan, dtype, mode);
   if (plan->work_bytes < L.total) return fail(QLT_ERR_INTERNAL, "split workspace too small");
+  if (L.exp_elems >= (int64_t{1} << 31))   // task records hold 32-bit exponent offsets
+    return fail(QLT_ERR_UNSUPPORTED, "split workspace: more than 2^31 packed rows");
   char *base = static_cast<char *>(plan->d_work);
   std::vector<int64_t> aoffs(plan->n_a_blks, 0), boffs(plan->n_b_blks, 0);
   for (const ExecTask &e : plan->h_exec) {

@@ -143,6 +143,22 @@ def test_split_beta_and_alpha(forced_split, split_mode):
     assert rel_frobenius(got, exp) <= TOL
 
 
+def test_split_real_alpha_beta_complex(forced_split, split_mode):
+    """Real alpha and beta on complex128: the int8 kernel's waves then store
+    their own output component (beta read back per component)."""
+    rng = np.random.default_rng(6)
+    a, b, a_ax, b_ax = random_case(rng, dtype="c128", allow_scalar=False, max_deg=40, min_deg=8)
+    plan = ContractionPlan(a, b, a_ax, b_ax)
+    n = plan.info.c_raw_size
+    init = (np.arange(n) % 7 - 3.0) + 1j * (np.arange(n) % 5 - 2.0)
+    da, db = capi.DeviceBuffer.from_numpy(a.data), capi.DeviceBuffer.from_numpy(b.data)
+    dc = capi.DeviceBuffer.from_numpy(init.astype(np.complex128))
+    plan.execute("c128", split_mode, da.ptr, db.ptr, dc.ptr, alpha=0.5, beta=2.0)
+    got = dc.to_numpy(np.complex128)
+    exp = 0.5 * O.OraclePlan(a, b, a_ax, b_ax).execute() + 2.0 * init
+    assert rel_frobenius(got, exp) <= TOL
+
+
 @pytest.mark.parametrize("u1u1,dtype", [(False, "f64"), (True, "c128")])
 def test_split_small_matvec_chain(forced_split, split_mode, u1u1, dtype):
     mv = W.small_matvec(D=64 if u1u1 else 96, dtype=dtype, u1u1=u1u1)
