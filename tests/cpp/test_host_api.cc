@@ -357,8 +357,8 @@ int main() {
     CheckContract(a, b, {1}, {0}, "after cache hit");
   }
 
-  // ---- split-BF16 precision mode keeps 1e-12 ----
-  {
+  // ---- split precision modes (BF16 slices, int8 digits) keep 1e-12 ----
+  for (auto prec : {hp_numeric::ContractPrecision::kSplitBF16, hp_numeric::ContractPrecision::kSplitInt8}) {
     QNSectorVec<U1QN> big = {QNSctT(U1QN(-1), 70), QNSctT(U1QN(0), 90), QNSctT(U1QN(1), 80)};
     U1Idx bi(big, IN), bo(big, OUT);
     ZTen a({bi, bo});
@@ -368,12 +368,12 @@ int main() {
     ZTen cn, cs;
     Contract(&a, &b, {{1}, {0}}, &cn);
     QltCheck(qlt_set_split_min_mean_k(0.0));
-    hp_numeric::SetContractPrecision(hp_numeric::ContractPrecision::kSplitBF16);
+    hp_numeric::SetContractPrecision(prec);
     Contract(&a, &b, {{1}, {0}}, &cs);
     hp_numeric::SetContractPrecision(hp_numeric::ContractPrecision::kNativeFP64);
     QltCheck(qlt_set_split_min_mean_k(48.0));
     const double err = RelDiff(Densify(cs), Densify(cn));
-    CHECK(err < 1e-12, "split mode rel err " + std::to_string(err));
+    CHECK(err < 1e-12, "split mode " + std::to_string(static_cast<int>(prec)) + " rel err " + std::to_string(err));
   }
 
   // ---- element access, norms, linear combination ----
