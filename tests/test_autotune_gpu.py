@@ -86,7 +86,7 @@ def test_unsliced_target_keeps_one_tile_per_output_tile(builtin_tuning, dtype):
     assert per_block_rel_err(c1, exp) <= TOL
 
 
-@pytest.mark.parametrize("mode", [capi.QLT_MODE_NATIVE, capi.QLT_MODE_SPLIT_BF16])
+@pytest.mark.parametrize("mode", [capi.QLT_MODE_NATIVE, capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8])
 def test_autotune_small_matvec(builtin_tuning, mode):
     _say(f"case: autotune small U(1) matvec D=96 mode {mode}")
     mv = W.small_matvec(D=96)

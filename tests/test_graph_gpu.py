@@ -91,6 +91,7 @@ def test_graph_replay_native(qlt):
     _replays(capi.QLT_MODE_NATIVE)
 
 
-def test_graph_replay_split(qlt, forced_split):
-    print("case: graph replay, split forced, small U(1)xU(1) matvec", flush=True)
-    _replays(capi.QLT_MODE_SPLIT_BF16)
+@pytest.mark.parametrize("mode", [capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8], ids=["bf16", "i8"])
+def test_graph_replay_split(qlt, forced_split, mode):
+    print(f"case: graph replay, split mode {mode} forced, small U(1)xU(1) matvec", flush=True)
+    _replays(mode)

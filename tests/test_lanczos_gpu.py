@@ -29,10 +29,11 @@ def test_lanczos_native_matches_cpu(qlt, u1u1, dtype, D):
     assert abs(abs(np.vdot(gs_ref, gs)) - 1.0) <= 1e-9
 
 
-def test_lanczos_split_matches_cpu(qlt, forced_split):
-    print("case: lanczos split u1u1 D=64", flush=True)
+@pytest.mark.parametrize("mode", [capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8], ids=["bf16", "i8"])
+def test_lanczos_split_matches_cpu(qlt, forced_split, mode):
+    print(f"case: lanczos split mode {mode} u1u1 D=64", flush=True)
     mv = W.hermitian_matvec(W.small_matvec(D=64, dtype="c128", u1u1=True))
-    res, gs = _run(mv, capi.QLT_MODE_SPLIT_BF16, 12)
+    res, gs = _run(mv, mode, 12)
     theta, gs_ref, *_ = OL.lanczos(mv, mv.psi.data.copy(), 12)
     assert abs(res.eigenvalue - theta) <= 1e-11 * abs(theta)
     assert abs(abs(np.vdot(gs_ref, gs)) - 1.0) <= 1e-9
