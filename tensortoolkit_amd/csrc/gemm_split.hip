@@ -375,10 +375,11 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
 // workgroup of 256 threads per (side, block, group of RW rows) stages the
 // rows' whole k extent in LDS (the block is read once), then the KW / 8
 // threads of a row take the row maximum and split 8 consecutive k each.  KW
-// (64, 128 or 256, at least k_pad; carried in the item's k0 field) and RW =
-// 2048 / KW keep every lane busy on short blocks.  LDS rows hold one pad
-// double per 8 so that the 8-element runs are read without bank conflicts.
-constexpr int kShortK = 256;
+// (64, 128, 256 or 512, at least k_pad; carried in the item's k0 field) and
+// RW = 2048 / KW keep every lane busy on short blocks; a row has at most one
+// wave.  LDS rows hold one pad double per 8 so that the 8-element runs are
+// read without bank conflicts.
+constexpr int kShortK = 512;
 template <bool CPLX, bool I8>
 __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__ items,
                                                     const PackMeta *__restrict__ meta_a,
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
     const int r = it.r0 + tid;
     s_roff[tid] = (r < pm.rows) ? v.row_off(r) : -1;
   }
-  if (tid < KW) s_coff[tid] = (tid < pm.cols) ? v.col_off(tid) : -1;
+  for (int p = tid; p < KW; p += 256) s_coff[p] = (p < pm.cols) ? v.col_off(p) : -1;
   __syncthreads();
 
   // ---- one read of the RW rows (zeros outside the block) ----
@@ -1446,7 +1447,8 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
       L.plane_elems += static_cast<int64_t>(parts) * fmt.slices * m.r_pad * m.k_pad;
       L.exp_elems += m.r_pad;
       if (m.k_pad <= kShortK) {
-        const int32_t kw = m.k_pad <= 64 ? 64 : (m.k_pad <= 128 ? 128 : 256), rw = 2048 / kw;
+        const int32_t kw = m.k_pad <= 64 ? 64 : (m.k_pad <= 128 ? 128 : (m.k_pad <= 256 ? 256 : 512));
+        const int32_t rw = 2048 / kw;
         for (int32_t r0 = 0; r0 < m.r_pad; r0 += rw) L.sitems.push_back(PackItem{side, static_cast<int32_t>(i), r0, kw});
         continue;
       }

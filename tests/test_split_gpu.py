@@ -66,6 +66,22 @@ def test_split_long_k_many_flushes(forced_split, split_mode, dtype):
         assert err <= TOL, (axes, err)
 
 
+@pytest.mark.parametrize("k", [40, 64, 100, 256, 300, 512, 513])
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_pack_widths(forced_split, split_mode, dtype, k):
+    """Every single-read pack width (k up to 64, 128, 256, 512) at and around
+    its limits, and the two-pass pack beyond 512; ragged rows, transposed B."""
+    ix_m = Index(((0,),), (77,), IN)
+    ix_k = Index(((0,),), (k,), OUT)
+    ix_n = Index(((0,),), (45,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix_m, ix_k], (0,), dtype).fill_uniform(61)
+    b = BlockSparseTensor.allowed_blocks([ix_n.inverse(), ix_k.inverse()], (0,), dtype).fill_uniform(62)
+    _say(f"case: split pack width mode={split_mode} {dtype} k={k}")
+    plan, c = gpu_contract(a, b, [1], [1], mode=split_mode)
+    exp = O.OraclePlan(a, b, [1], [1]).execute()
+    assert rel_frobenius(c.data, exp) <= TOL
+
+
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_split_k_beyond_int8_flush_interval(forced_split, split_mode, dtype):
     """k = 8500 crosses the int8 form's 8192-k flush of its int32 digit
