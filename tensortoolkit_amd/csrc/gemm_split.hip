@@ -1036,7 +1036,9 @@ constexpr double k255p5 = 1078203909375.0;   // 255^5, exact in FP64
 // 2 x 2 waves of 32 x 32, the 21 digit pairs of a 32-k chunk as one
 // v_mfma_i32_32x32x32_i8 each, with the software pipeline of
 // k_split_gemm_f64 (the next chunk's digits in flight while the MFMAs run).
-__global__ __launch_bounds__(256) void k_split_i8_f64(
+// Bounded to two workgroups per CU: the registers then fit two waves per
+// SIMD (1.8x faster than one wave per SIMD with the level sums in AGPRs).
+__global__ __launch_bounds__(256, 2) void k_split_i8_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
     const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
