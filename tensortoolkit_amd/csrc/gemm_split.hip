@@ -1,17 +1,19 @@
 // Split-precision grouped block GEMM: FP64 (and complex128) emulated on the
-// BF16 MFMA units.
+// BF16 MFMA units, or on the int8 MFMA units (below).
 //
 // Same contract as the native kernel (it replaces the per-task MatMultiply
 // loop of CtrctTwoBSDTAndAssignIn, global_operations.h:895-992); only the
-// arithmetic differs.  Two launches per execute:
+// arithmetic differs.  Two steps per execute:
 //
-// 1. pack: every block the plan touches is read once, through the same gather
-//    descriptors as the native kernel (so the transposes stay folded), scaled
-//    per row of A' and per column of B' by a power of two 2^e with
-//    max|x| / 2^e in [1/2, 1], and split by round-to-nearest into S = 5 slices
+// 1. pack (k_pack_short for blocks with k_pad <= 512, which are read once;
+//    k_pack phases 1 and 2 for longer ones): every block the plan touches is
+//    read through the same gather descriptors as the native kernel (so the
+//    transposes stay folded), scaled per row of A' and per column of B' by a
+//    power of two 2^e with max|x| / 2^e in [1/2, 1], and split by
+//    round-to-nearest into S = 5 slices
 //        y = x / 2^e,  c1 = rint(y * 2^8),  r1 = y - c1 2^-8,
 //        c_s = rint(r_{s-1} * 2^(8 + 9(s-1))),  |c_s| <= 256,
-//    so y = 2^-8 * sum_s c_s 2^(-9(s-1)) + O(2^-53).  Slice s is stored as the
+//    so y = 2^-8 * sum_s c_s 2^(-9(s-1)) + r_5 with |r_5| <= 2^-45.  Slice s is stored as the
 //    BF16 number c_s 2^(-9(s-1)) (integers |c| <= 256 have at most 8
 //    significant bits, so every slice is exact in BF16).
 // 2. GEMM: C' = sum over slice pairs with sa + sb <= S + 1 (15 products) of
