@@ -1,7 +1,7 @@
 """Lanczos ground state of the two-site effective Hamiltonian on the device.
 
 The matvec psi' = L . psi . W1 . W2 . R runs as four grouped contractions on
-cached device plans (in native FP64 or split-BF16 mode); the Krylov vectors
+cached device plans (native FP64, split-BF16 or split-int8 mode); the Krylov vectors
 stay in HBM and every vector operation is a device kernel in FP64 -- the
 reference's LinearCombine (ten_linear_combine.h:186), InnerProduct
 (basic_operations.h:203) and Normalize (qltensor_impl.h:568-573) building
