@@ -5,7 +5,7 @@
 // loop of CtrctTwoBSDTAndAssignIn, global_operations.h:895-992); only the
 // arithmetic differs.  Two steps per execute:
 //
-// 1. pack (k_pack_short for blocks with k_pad <= 512, which are read once;
+// 1. pack (k_pack_short for blocks with k_pad <= 1024, which are read once;
 //    k_pack phases 1 and 2 for longer ones): every block the plan touches is
 //    read through the same gather descriptors as the native kernel (so the
 //    transposes stay folded), scaled per row of A' and per column of B' by a
@@ -377,11 +377,11 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
 // workgroup of 256 threads per (side, block, group of RW rows) stages the
 // rows' whole k extent in LDS (the block is read once), then the KW / 8
 // threads of a row take the row maximum and split 8 consecutive k each.  KW
-// (64, 128, 256 or 512, at least k_pad; carried in the item's k0 field) and
-// RW = 2048 / KW keep every lane busy on short blocks; a row has at most one
-// wave.  LDS rows hold one pad double per 8 so that the 8-element runs are
-// read without bank conflicts.
-constexpr int kShortK = 512;
+// (64, 128, 256, 512 or 1024, at least k_pad; carried in the item's k0 field)
+// and RW = 2048 / KW keep every lane busy on short blocks; a row of 1024 k
+// spans two waves, whose maxima meet in LDS.  LDS rows hold one pad double
+// per 8 so that the 8-element runs are read without bank conflicts.
+constexpr int kShortK = 1024;
 template <bool CPLX, bool I8>
 __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__ items,
                                                     const PackMeta *__restrict__ meta_a,
@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
   __shared__ int32_t s_roff[ELEMS / 64];
   __shared__ int32_t s_coff[kShortK];
   __shared__ double s_tile[PARTS][ELEMS + ELEMS / 8 + 32];
+  __shared__ double s_wmax[4];
 
   const PackItem it = items[blockIdx.x];
   const int KW = it.k0, RW = ELEMS / KW, TPR = KW / RUN, lrow = KW + KW / 8 + 1;
@@ -452,7 +453,14 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
       x[part][u] = s_tile[part][r * lrow + 9 * run + u];
       m = fmax(m, fabs(x[part][u]));
     }
-  for (int o = TPR / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, TPR));
+  if (TPR <= 64) {
+    for (int o = TPR / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, TPR));
+  } else {                                       // two waves per row
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((tid & 63) == 0) s_wmax[tid >> 6] = m;
+    __syncthreads();
+    m = fmax(s_wmax[2 * r], s_wmax[2 * r + 1]);
+  }
   int ex = 0;
   if (m > 0.0) (void)frexp(m, &ex);
   const int grow = it.r0 + r;                    // < r_pad
@@ -1451,7 +1459,7 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
       L.plane_elems += static_cast<int64_t>(parts) * fmt.slices * m.r_pad * m.k_pad;
       L.exp_elems += m.r_pad;
       if (m.k_pad <= kShortK) {
-        const int32_t kw = m.k_pad <= 64 ? 64 : (m.k_pad <= 128 ? 128 : (m.k_pad <= 256 ? 256 : 512));
+        const int32_t kw = m.k_pad <= 64 ? 64 : m.k_pad <= 128 ? 128 : m.k_pad <= 256 ? 256 : m.k_pad <= 512 ? 512 : 1024;
         const int32_t rw = 2048 / kw;
         for (int32_t r0 = 0; r0 < m.r_pad; r0 += rw) L.sitems.push_back(PackItem{side, static_cast<int32_t>(i), r0, kw});
         continue;

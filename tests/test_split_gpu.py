@@ -66,11 +66,12 @@ def test_split_long_k_many_flushes(forced_split, split_mode, dtype):
         assert err <= TOL, (axes, err)
 
 
-@pytest.mark.parametrize("k", [40, 64, 100, 256, 300, 512, 513])
+@pytest.mark.parametrize("k", [40, 64, 100, 256, 300, 512, 513, 1024, 1025])
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_split_pack_widths(forced_split, split_mode, dtype, k):
-    """Every single-read pack width (k up to 64, 128, 256, 512) at and around
-    its limits, and the two-pass pack beyond 512; ragged rows, transposed B."""
+    """Every single-read pack width (k up to 64, 128, 256, 512, 1024) at and
+    around its limits, and the two-pass pack beyond 1024; ragged rows,
+    transposed B."""
     ix_m = Index(((0,),), (77,), IN)
     ix_k = Index(((0,),), (k,), OUT)
     ix_n = Index(((0,),), (45,), OUT)
