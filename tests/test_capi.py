@@ -30,6 +30,30 @@ def test_binding_covers_every_declared_symbol():
     assert sorted(capi.EXPORTED_SYMBOLS) == declared_functions()
 
 
+def declared_constants():
+    """Name -> value of the header's enum constants and integer #defines."""
+    text = (ROOT / "include" / "qlten_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, flags=re.S):
+        for name, val in re.findall(r"(QLT_[A-Z0-9_]+)\s*=\s*(-?\d+)", body):
+            out[name] = int(val)
+    for name, val in re.findall(r"#define\s+(QLT_[A-Z0-9_]+)\s+(\d+)", text):
+        out[name] = int(val)
+    return out
+
+
+def test_binding_constants_match_the_header():
+    consts = declared_constants()
+    for name in ("QLT_OK", "QLT_F64", "QLT_C128", "QLT_MODE_NATIVE", "QLT_MODE_SPLIT_BF16",
+                 "QLT_MODE_SPLIT_I8", "QLT_MAX_RANK", "QLT_TUNE_BUCKETS", "QLT_TILE_CLASSES"):
+        assert name in consts, name
+    for name, val in consts.items():
+        if hasattr(capi, name):
+            assert getattr(capi, name) == val, (name, getattr(capi, name), val)
+    assert len({consts[m] for m in ("QLT_MODE_NATIVE", "QLT_MODE_SPLIT_BF16", "QLT_MODE_SPLIT_I8")}) == 3
+
+
 def test_task_record_layout_matches_reference_fields():
     # RawDataCtrctTask (raw_data_operation_tasks.h:196-208): nine size_t fields
     # then f_ex_sign and beta.
