@@ -769,13 +769,13 @@ __global__ __launch_bounds__(256) void k_split_gemm_f64(
 // while the MFMAs consume one chunk from LDS, the next chunk's planes are in
 // flight into registers, also across a tile boundary, so a tile's metadata
 // and first chunk load behind the previous tile's last chunk.  When a tile
-// ends, the imaginary parts reach the real-part waves through a separate LDS
-// exchange area and the real-part waves store the tile.
+// ends, each wave stores its own output component.  The scalars are real: a
+// complex alpha or beta is applied around the kernel (launch_split).
 __global__ __launch_bounds__(512) void k_split_gemm_c2(
     const Tile *__restrict__ tiles, int32_t n_tiles, const Seg *__restrict__ segs,
     const ExecTask *__restrict__ exec, const PackMeta *__restrict__ meta_a, const PackMeta *__restrict__ meta_b,
-    const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
-    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+    const uint16_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C, double alpha,
+    double beta) {
   constexpr int PARTS = 2;
   constexpr int CHUNKS = PARTS * S * 128;          // 16-byte pieces per operand per k-chunk
   constexpr int NT = 512;
@@ -783,7 +783,6 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
   __shared__ __attribute__((aligned(16))) uint16_t As[2][PARTS][S][TM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PARTS][S][TN][LDK];
   __shared__ int32_t s_ea[2][TM];
-  __shared__ double xch[4][16][64];                 // imaginary parts of a finished tile
 
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ws = w & 3, comp = w >> 2;             // spatial block, output component
@@ -888,7 +887,6 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
     if constexpr (NG >= 3) g[1] = v16f{};
     if constexpr (NG >= 4) g[0] = v16f{};
   };
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
 
   while (true) {
     // ---- the next chunk: same task, next task, or the next tile's first ----
@@ -961,29 +959,18 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
         }
       }
     }
-    // ---- a finished tile: imaginary parts to the real-part waves, store ----
+    // ---- a finished tile: each wave stores its own output component ----
     if (tile_end) {
-      if (comp == 1) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) xch[ws][r][lane] = acc64[r];
-      }
-      __syncthreads();
-      if (comp == 0 && active) {
+      if (active) {
         const int col = cur.j0 + wn + lrow;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
           if (row < cur.seg.m && col < cur.seg.n) {
-            const int64_t idx = cur.seg.c_off + static_cast<int64_t>(row) * cur.seg.n + col;
-            const double xr = acc64[r], xi = xch[ws][r][lane];
-            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
-            if (use_beta) {
-              const double2 o = *cp;
-              yr += beta_r * o.x - beta_i * o.y;
-              yi += beta_r * o.y + beta_i * o.x;
-            }
-            *cp = double2{yr, yi};
+            double *cp = C + 2 * (cur.seg.c_off + static_cast<int64_t>(row) * cur.seg.n + col) + comp;
+            double y = alpha * acc64[r];
+            if (beta != 0.0) y += beta * *cp;
+            *cp = y;
           }
         }
       }
@@ -1184,12 +1171,12 @@ __global__ __launch_bounds__(256, 2) void k_split_i8_f64(
 // 512-thread workgroup per CU walking tiles blockIdx.x, blockIdx.x + gridDim.x,
 // ...; waves 0-3 the real part ar*br + (-ai)*bi, waves 4-7 the imaginary part
 // ar*bi + ai*br of the same 2 x 2 arrangement of 32 x 32 blocks; the pipeline
-// crosses task and tile boundaries; the imaginary parts of a finished tile go
-// to the real-part waves through an LDS exchange area) on int8 digits.
+// crosses task and tile boundaries; each wave stores its own component of a
+// finished tile; real scalars) on int8 digits.
 __global__ __launch_bounds__(512) void k_split_i8_c2(
     const SplitTile *__restrict__ stiles, int32_t n_tiles, const SplitTask *__restrict__ stasks,
-    const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C,
-    double alpha_r, double alpha_i, double beta_r, double beta_i) {
+    const int8_t *__restrict__ planes, const int32_t *__restrict__ exps, double *__restrict__ C, double alpha,
+    double beta) {
   constexpr int PARTS = 2;
   constexpr int PIECES = PARTS * S8 * 128;          // 16-byte pieces per operand per k-chunk
   constexpr int NT = 512;
@@ -1198,7 +1185,6 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
   __shared__ __attribute__((aligned(16))) int8_t As[2][PARTS][S8][TM][KC8];
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][PARTS][S8][TN][KC8];
   __shared__ int32_t s_ea[2][TM];
-  __shared__ double xch[4][16][64];                 // imaginary parts of a finished tile
 
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ws = w & 3, comp = w >> 2;             // spatial block, output component
@@ -1263,8 +1249,6 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
   stage(0);
   __syncthreads();
   int buf = 0;
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
-  const bool real_scalars = (alpha_i == 0.0) && (beta_i == 0.0);
 
   while (true) {
     const bool new_task = kc + 1 == tk.nkc;
@@ -1323,8 +1307,8 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
         for (int l = 0; l < S8; ++l) g[l] = v16i{};
       }
     }
-    if (tile_end && real_scalars) {
-      // each wave stores its own component (alpha and beta real)
+    // ---- a finished tile: each wave stores its own output component ----
+    if (tile_end) {
       if (active) {
         const int col = cur.j0 + wn + lrow;
 #pragma unroll
@@ -1332,36 +1316,9 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
           const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
           if (row < cur.m && col < cur.n) {
             double *cp = C + 2 * (cur.c_off + static_cast<int64_t>(row) * cur.n + col) + comp;
-            double y = alpha_r * acc64[r];
-            if (use_beta) y += beta_r * *cp;
+            double y = alpha * acc64[r];
+            if (beta != 0.0) y += beta * *cp;
             *cp = y;
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
-    } else if (tile_end) {
-      if (comp == 1) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) xch[ws][r][lane] = acc64[r];
-      }
-      __syncthreads();
-      if (comp == 0 && active) {
-        const int col = cur.j0 + wn + lrow;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int row = cur.i0 + wm + 8 * (r >> 2) + 4 * lh + (r & 3);
-          if (row < cur.m && col < cur.n) {
-            const int64_t idx = cur.c_off + static_cast<int64_t>(row) * cur.n + col;
-            const double xr = acc64[r], xi = xch[ws][r][lane];
-            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
-            if (use_beta) {
-              const double2 o = *cp;
-              yr += beta_r * o.x - beta_i * o.y;
-              yi += beta_r * o.y + beta_i * o.x;
-            }
-            *cp = double2{yr, yi};
           }
         }
       }
@@ -1392,6 +1349,22 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
     }
     kc = kc2;
     buf ^= 1;
+  }
+}
+
+// Scales the tiles the complex split kernels write by a complex factor: a
+// complex alpha or beta is applied around those kernels, which take real
+// scalars, as C = alpha (X + (beta / alpha) C).
+__global__ __launch_bounds__(256) void k_scale_tiles(const SplitTile *__restrict__ tiles, double sr, double si,
+                                                     double *__restrict__ C) {
+  const SplitTile t = tiles[blockIdx.x];
+  for (int e = static_cast<int>(threadIdx.x); e < TM * TN; e += 256) {
+    const int row = t.i0 + e / TN, col = t.j0 + e % TN;
+    if (row < t.m && col < t.n) {
+      double2 *p = reinterpret_cast<double2 *>(C) + t.c_off + static_cast<int64_t>(row) * t.n + col;
+      const double2 v = *p;
+      *p = double2{sr * v.x - si * v.y, sr * v.y + si * v.x};
+    }
   }
 }
 
@@ -1701,21 +1674,54 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const
     if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_gemm_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
                        meta_b, planes16, exps, cd, ar, br);
-  } else if (cplx && !i8) {
-    launch_pack(k_pack_short<true, false>, k_pack<true, false, 1>, k_pack<true, false, 2>);
-    if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
-                       plan->d_segs, plan->d_exec, meta_a, meta_b, planes16, exps, cd, ar, ai, br, bi);
   } else if (!cplx) {
     launch_pack(k_pack_short<false, true>, k_pack<false, true, 1>, k_pack<false, true, 2>);
     if (probe_split) probe_begin(st);
     hipLaunchKernelGGL(k_split_i8_f64, dim3(n_mfma), blk, 0, st, stiles, plan->d_segs, plan->d_exec, meta_a,
                        meta_b, planes8, exps, cd, ar, br);
   } else {
-    launch_pack(k_pack_short<true, true>, k_pack<true, true, 1>, k_pack<true, true, 2>);
-    if (probe_split) probe_begin(st);
-    hipLaunchKernelGGL(k_split_i8_c2, dim3(grid), dim3(512), 0, st, rtiles, static_cast<int32_t>(n_mfma), rtasks,
-                       planes8, exps, cd, ar, ai, br, bi);
+    // The complex kernels take real scalars; a complex alpha or beta is
+    // applied on their tiles around them: C = alpha (X + (beta / alpha) C),
+    // C = alpha X when beta = 0 (C is not read), C = beta C when alpha = 0.
+    double gar = ar, gbr = br;
+    bool pre = false, post = false, gemm = true;
+    double pre_r = 0.0, pre_i = 0.0;
+    if (ai != 0.0 || bi != 0.0) {
+      if (ar == 0.0 && ai == 0.0) {
+        gemm = false;
+        pre = true;
+        pre_r = br;
+        pre_i = bi;
+      } else if (br == 0.0 && bi == 0.0) {
+        gar = 1.0;
+        post = true;
+      } else {
+        const double d = ar * ar + ai * ai;
+        pre = true;
+        pre_r = (br * ar + bi * ai) / d;
+        pre_i = (bi * ar - br * ai) / d;
+        gar = 1.0;
+        gbr = 1.0;
+        post = true;
+      }
+    }
+    if (pre) hipLaunchKernelGGL(k_scale_tiles, dim3(n_mfma), blk, 0, st, rtiles, pre_r, pre_i, cd);
+    if (gemm) {
+      if (i8)
+        launch_pack(k_pack_short<true, true>, k_pack<true, true, 1>, k_pack<true, true, 2>);
+      else
+        launch_pack(k_pack_short<true, false>, k_pack<true, false, 1>, k_pack<true, false, 2>);
+      if (probe_split) probe_begin(st);
+      if (i8)
+        hipLaunchKernelGGL(k_split_i8_c2, dim3(grid), dim3(512), 0, st, rtiles, static_cast<int32_t>(n_mfma),
+                           rtasks, planes8, exps, cd, gar, gbr);
+      else
+        hipLaunchKernelGGL(k_split_gemm_c2, dim3(grid), dim3(512), 0, st, stiles, static_cast<int32_t>(n_mfma),
+                           plan->d_segs, plan->d_exec, meta_a, meta_b, planes16, exps, cd, gar, gbr);
+    } else if (probe_split) {
+      probe_begin(st);
+    }
+    if (post) hipLaunchKernelGGL(k_scale_tiles, dim3(n_mfma), blk, 0, st, rtiles, ar, ai, cd);
   }
   if (probe_split) {
     double elems = 0.0;
