@@ -160,6 +160,31 @@ def test_split_beta_and_alpha(forced_split, split_mode):
     assert rel_frobenius(got, exp) <= TOL
 
 
+@pytest.mark.parametrize("alpha,beta", [(0.5 - 1.5j, 0.0), (0.0, 2.0 + 0.25j), (1.0, 0.5j)],
+                         ids=["complex-alpha-beta0", "alpha0-complex-beta", "real-alpha-imag-beta"])
+def test_split_complex_scalar_special_cases(forced_split, split_mode, alpha, beta):
+    """The complex split kernels take real scalars; a complex alpha or beta is
+    applied on their tiles around them.  beta = 0 must not read C (it holds
+    NaN here), alpha = 0 only scales C."""
+    rng = np.random.default_rng(8)
+    a, b, a_ax, b_ax = random_case(rng, dtype="c128", allow_scalar=False, max_deg=40, min_deg=8)
+    plan = ContractionPlan(a, b, a_ax, b_ax)
+    n = plan.info.c_raw_size
+    if beta == 0:
+        init = np.full(n, np.nan + 1j * np.nan)
+    else:
+        init = (np.arange(n) % 7 - 3.0) + 1j * (np.arange(n) % 5 - 2.0)
+    da, db = capi.DeviceBuffer.from_numpy(a.data), capi.DeviceBuffer.from_numpy(b.data)
+    dc = capi.DeviceBuffer.from_numpy(init.astype(np.complex128))
+    plan.execute("c128", split_mode, da.ptr, db.ptr, dc.ptr, alpha=alpha, beta=beta)
+    got = dc.to_numpy(np.complex128)
+    exp = alpha * O.OraclePlan(a, b, a_ax, b_ax).execute()
+    if beta != 0:
+        exp = exp + beta * init
+    assert np.all(np.isfinite(got))
+    assert rel_frobenius(got, exp) <= TOL
+
+
 def test_split_real_alpha_beta_complex(forced_split, split_mode):
     """Real alpha and beta on complex128: the int8 kernel's waves then store
     their own output component (beta read back per component)."""
