@@ -6,10 +6,11 @@
 // (hp_numeric::TensorTranspose, framework/hp_numeric/ten_trans.h:56-114,245-349)
 // and then issues one BLAS call C_blk = A'(m x k) B'(k x n) + beta C_blk
 // (hp_numeric::MatMultiply, framework/hp_numeric/blas_level3.h:35-56,798-856).
-// Here one launch covers the whole plan.  Each workgroup owns one 64 x 64
-// tile of one output block and walks every task of that block, so the
-// per-block beta chain (0 for the first task, 1 after) becomes a single
-// write of the finished tile.  The transposes are never materialised: the
+// Here one launch per tile class covers the whole plan.  Each workgroup owns
+// one TM x TN tile (64 or 32 each way) of one output block, or one k-slice of
+// it, and walks every task of that block, so the per-block beta chain (0 for
+// the first task, 1 after) becomes a single write of the finished tile (or of
+// its slices' sum).  Narrow outputs (n <= 16) go to a row-streaming kernel.  The transposes are never materialised: the
 // prologue gathers A'(i, p) = A[a_off + row_off(i) + col_off(p)] straight
 // from the original block layout through per-tile offset tables, staging the
 // k-chunk in LDS, with the thread mapping chosen so that the block's
