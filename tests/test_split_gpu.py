@@ -50,6 +50,25 @@ def test_split_random(forced_split, split_mode, seed, dtype):
     assert err <= TOL, err
 
 
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_random_rank4(forced_split, split_mode, seed, dtype):
+    """Ranks up to 4 with contracted axes in random order: the gathers of
+    permuted, non-plain blocks feed the split pack."""
+    rng = np.random.default_rng(9500 + seed)
+    while True:   # draw until the case has output blocks wide enough for the split kernels
+        a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1 + seed % 2, max_rank=4, max_sectors=3,
+                                       max_deg=20, min_deg=6, allow_scalar=False)
+        t = O.OraclePlan(a, b, a_ax, b_ax).tasks()
+        if len(t) and (t["n"] > 16).any() and (t["m"] > 16).any():
+            break
+    _say(f"case: split rank4 mode={split_mode} seed={seed} {dtype} ranks={a.rank},{b.rank} axes={a_ax},{b_ax}")
+    plan, c = gpu_contract(a, b, a_ax, b_ax, mode=split_mode)
+    ref = O.OraclePlan(a, b, a_ax, b_ax)
+    err = per_block_rel_err(c, ref.execute())
+    assert err <= TOL, err
+
+
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_split_long_k_many_flushes(forced_split, split_mode, dtype):
     """k = 3 * 200 = 600 spans many FP32 flush intervals; ragged m, n."""
