@@ -381,7 +381,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the contraction would shard by output block, see DESIGN.md)", file=sys.stderr)
+              "(the contraction would shard by output block across GPUs)", file=sys.stderr)
         return 2
 
     if args.e2e:

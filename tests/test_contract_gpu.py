@@ -2,7 +2,7 @@
 
 Plan: bit-exact (every RawDataCtrctTask field, the output block map).
 Values: relative Frobenius error of every output block <= 1e-12 against the
-FP64 oracle, for FP64 and complex128.
+FP64 oracle (the required accuracy), for FP64 and complex128.
 """
 import numpy as np
 import pytest

@@ -2,7 +2,7 @@
 digits, against the FP64 oracle.
 
 Every output block must be within 1e-12 relative Frobenius error of the
-oracle, and the plan is the same plan as the native
+oracle (the required accuracy), and the plan is the same plan as the native
 path.  The split kernel is forced (dispatch threshold 0) so that small test
 cases exercise it rather than the native fallback for low-k contractions.
 """
