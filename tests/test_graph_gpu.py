@@ -39,11 +39,13 @@ def _chain(mv, mode, stream):
     return plans, bufs, outs, run, x
 
 
-def _replays(mode):
+def _replays(mode, mv=None):
     lib = capi.lib()
     stream = C.c_void_p()
     capi.check(lib.qlt_stream_create(C.byref(stream)))
-    mv = W.small_matvec(D=64, dtype="c128", u1u1=True)
+    if mv is None:
+        mv = W.small_matvec(D=64, dtype="c128", u1u1=True)
+    assert mv.dtype == "c128"
     plans, bufs, outs, run, out_t = _chain(mv, mode, stream)
     npd = np.complex128
 
@@ -95,3 +97,13 @@ def test_graph_replay_native(qlt):
 def test_graph_replay_split(qlt, forced_split, mode):
     print(f"case: graph replay, split mode {mode} forced, small U(1)xU(1) matvec", flush=True)
     _replays(mode)
+
+
+@pytest.mark.parametrize("mode", [capi.QLT_MODE_NATIVE, capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8],
+                         ids=["native", "bf16", "i8"])
+def test_graph_replay_full_size(qlt, mode):
+    """The D = 4096 U(1)xU(1) complex128 chain at the default dispatch: k-sliced
+    tile classes with their partial-sum reductions, the split GEMMs on L.psi
+    and every class stream forked inside the capture."""
+    print(f"case: graph replay, mode {mode}, D = 4096 U(1)xU(1) complex128 matvec", flush=True)
+    _replays(mode, W.config3())
