@@ -13,5 +13,6 @@
 #include "qlten/tensor_manipulation/ten_ctrct.h"
 #include "qlten/tensor_manipulation/ten_decomp/ten_qr.h"
 #include "qlten/tensor_manipulation/ten_decomp/ten_svd.h"
+#include "qlten/tensor_manipulation/tensor_op_cost.h"
 
 #endif

@@ -14,7 +14,9 @@
 #include <functional>
 #include <sstream>
 #include <streambuf>
+#include <map>
 #include <numeric>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -215,6 +217,120 @@ DTen d2_for_fail(const Fixture &f) {
   DTen d2({f.idx_in_s, f.idx_out_s});
   d2.Random(U1QN(0));
   return d2;
+}
+
+// EstimateContractCost (tensor_op_cost.h:434-530) against counts derived on
+// the host from the operands' block maps alone: blocks meet when their
+// contracted sector coordinates agree; the first task of an output block does
+// not read it; a transposed operand contributes its distinct touched blocks.
+// The output layout must be the one Contract produces.
+template <typename T>
+void CheckContractCost(const QLTensor<T, U1QN> &a, const QLTensor<T, U1QN> &b, const std::vector<size_t> &a_ax,
+                       const std::vector<size_t> &b_ax, bool a_trans, bool b_trans, const std::string &name) {
+  const auto cost = EstimateContractCost(a, b, {a_ax, b_ax});
+  std::vector<size_t> a_sv, b_sv;
+  for (size_t i = 0; i < a.Rank(); ++i)
+    if (std::find(a_ax.begin(), a_ax.end(), i) == a_ax.end()) a_sv.push_back(i);
+  for (size_t i = 0; i < b.Rank(); ++i)
+    if (std::find(b_ax.begin(), b_ax.end(), i) == b_ax.end()) b_sv.push_back(i);
+  const auto &am = a.GetBlkSparDataTen().GetBlkIdxDataBlkMap();
+  const auto &bm = b.GetBlkSparDataTen().GetBlkIdxDataBlkMap();
+  const size_t esz = sizeof(T);
+  double flops = 0;
+  size_t pairs = 0, rd = 0, wr = 0;
+  std::map<CoorsT, size_t> c_blocks;   // output coordinates -> tasks seen
+  std::set<size_t> a_used, b_used;
+  for (const auto &ka : am)
+    for (const auto &kb : bm) {
+      bool meet = true;
+      for (size_t i = 0; i < a_ax.size(); ++i)
+        meet = meet && ka.second.blk_coors[a_ax[i]] == kb.second.blk_coors[b_ax[i]];
+      if (!meet) continue;
+      size_t m = 1, k = 1, n = 1;
+      CoorsT cc;
+      for (size_t ax : a_sv) m *= ka.second.shape[ax], cc.push_back(ka.second.blk_coors[ax]);
+      for (size_t ax : a_ax) k *= ka.second.shape[ax];
+      for (size_t ax : b_sv) n *= kb.second.shape[ax], cc.push_back(kb.second.blk_coors[ax]);
+      ++pairs;
+      flops += (std::is_same<T, QLTEN_Complex>::value ? 8.0 : 2.0) * double(m) * double(k) * double(n);
+      rd += (m * k + k * n) * esz;
+      if (c_blocks[cc]++ > 0) rd += m * n * esz;
+      wr += m * n * esz;
+      a_used.insert(ka.first);
+      b_used.insert(kb.first);
+    }
+  size_t temp = 0;
+  if (a_trans)
+    for (size_t i : a_used) temp += am.at(i).size * esz;
+  if (b_trans)
+    for (size_t i : b_used) temp += bm.at(i).size * esz;
+  CHECK(cost.flops == flops, name + ": flops");
+  CHECK(cost.gemm_count == pairs && cost.executable_block_pair_count == pairs, name + ": tasks");
+  CHECK(cost.candidate_block_pair_count == am.size() * bm.size(), name + ": candidate pairs");
+  CHECK(cost.output_block_count == c_blocks.size(), name + ": output blocks");
+  CHECK(cost.temp_peak_bytes == temp, name + ": temp bytes");
+  CHECK(cost.read_bytes == rd + temp, name + ": read bytes " + std::to_string(cost.read_bytes) + " vs " +
+                                          std::to_string(rd + temp));
+  CHECK(cost.write_bytes == wr + temp, name + ": write bytes");
+  QLTensor<T, U1QN> c;
+  Contract(&a, &b, {a_ax, b_ax}, &c);
+  CHECK(cost.output_layout == MakeBlockSparseTensorLayout(c), name + ": output layout");
+  CHECK(cost.output_raw_elem_count == c.GetBlkSparDataTen().RawDataSize(), name + ": output elements");
+  std::vector<std::pair<size_t, size_t>> pairs_form;
+  for (size_t i = 0; i < a_ax.size(); ++i) pairs_form.emplace_back(a_ax[i], b_ax[i]);
+  const auto cost2 = EstimateContractCost(a, b, pairs_form);
+  CHECK(cost2.flops == cost.flops && cost2.read_bytes == cost.read_bytes &&
+            cost2.output_layout == cost.output_layout, name + ": pair form");
+}
+
+void TestContractCost(const Fixture &f) {
+  // test_tensor_op_cost.cc:19-42: one (2 x 3) . (3 x 4) block pair
+  const U1Idx lf({QNSctT(f.qn0, 2)}, OUT), cl({QNSctT(f.qn0, 3)}, OUT), cr({QNSctT(f.qn0, 3)}, IN),
+      rf({QNSctT(f.qn0, 4)}, OUT);
+  DTen l({lf, cl}), r({cr, rf});
+  l.Fill(f.qn0, 1.0);
+  r.Fill(f.qn0, 1.0);
+  const auto c = EstimateContractCost(l, r, {{1}, {0}});
+  CHECK(c.flops == 48.0, "cost: flops 48");
+  CHECK(c.gemm_count == 1 && c.executable_block_pair_count == 1, "cost: one task");
+  CHECK(c.output_block_count == 1 && c.output_raw_elem_count == 8, "cost: one 8-element block");
+  CHECK(c.read_bytes == 18 * sizeof(QLTEN_Double) && c.write_bytes == 8 * sizeof(QLTEN_Double), "cost: bytes");
+  CHECK(c.temp_peak_bytes == 0, "cost: no transpose");
+  CHECK(c.output_layout.indexes == (IndexVec<U1QN>{lf, rf}), "cost: output indexes");
+  CHECK(c.output_layout.blocks.size() == 1 && c.output_layout.blocks.begin()->second.shape == (ShapeT{2, 4}),
+        "cost: output block shape");
+
+  // several sectors, accumulating output blocks, with and without transposes
+  DTen a({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+  DTen b({f.idx_in_s, f.idx_in_l, f.idx_out_s});
+  a.Random(U1QN(0));
+  b.Random(U1QN(1));
+  CheckContractCost(a, b, {2}, {0}, false, false, "cost rank3 tail-head");
+  CheckContractCost(a, b, {1, 2}, {1, 0}, false, true, "cost rank3 two axes, B permuted");
+  DTen b2({f.idx_in_l, f.idx_out_s, f.idx_in_s});
+  b2.Random(U1QN(0));
+  CheckContractCost(a, b2, {1}, {0}, true, false, "cost rank3 middle axis of A");
+  ZTen za({f.idx_in_s, f.idx_out_l, f.idx_out_s});
+  ZTen zb({f.idx_in_l, f.idx_in_s, f.idx_out_s});
+  za.Random(U1QN(0));
+  zb.Random(U1QN(-1));
+  CheckContractCost(za, zb, {2, 1}, {1, 0}, true, true, "cost complex, both permuted");
+
+  // errors: index mismatch, default operand
+  bool threw = false;
+  try {
+    EstimateContractCost(a, b, {{0}, {0}});
+  } catch (const std::invalid_argument &) {
+    threw = true;
+  }
+  CHECK(threw, "cost: index mismatch throws");
+  threw = false;
+  try {
+    EstimateContractCost(DTen(), b, {{0}, {0}});
+  } catch (const std::invalid_argument &) {
+    threw = true;
+  }
+  CHECK(threw, "cost: default operand throws");
 }
 
 int main() {
@@ -528,6 +644,9 @@ int main() {
       out2 << d3;
     }
   }
+
+  // ---- contraction cost estimate (tensor_op_cost.h) ----
+  TestContractCost(f);
 
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;
