@@ -353,6 +353,9 @@ class QLTensor {
   const Index<QNT> &GetIndex(const size_t i) const { return indexes_[i]; }
   bool IsDefault() const { return size_ == 0; }
   bool IsScalar() const { return rank_ == 0 && size_ == 1; }
+  // True once raw data is allocated (qltensor_impl.h:1071-1073); a tensor
+  // with blocks but no raw data is a layout shell.
+  bool HasActualData() const { return !IsDefault() && bsdt_ && bsdt_->GetActualRawDataSize() != 0; }
   BSDT &GetBlkSparDataTen() { return *bsdt_; }
   const BSDT &GetBlkSparDataTen() const { return *bsdt_; }
 

@@ -112,6 +112,20 @@ BlockSparseTensorLayout<QNT> MakeBlockSparseTensorLayout(const QLTensor<ElemT, Q
   return lay;
 }
 
+// A tensor with the layout's indexes and blocks and no raw data, so that the
+// estimate of one contraction can feed the next (tensor_op_cost.h:159-182).
+template <typename ElemT, typename QNT>
+QLTensor<ElemT, QNT> MakeTensorShellFromLayout(const BlockSparseTensorLayout<QNT> &layout) {
+  if (layout.IsDefault()) return QLTensor<ElemT, QNT>();
+  QLTensor<ElemT, QNT> shell(layout.indexes);
+  if (layout.IsScalar() || layout.blocks.empty()) return shell;
+  std::vector<CoorsT> coords;
+  coords.reserve(layout.blocks.size());
+  for (const auto &kv : layout.blocks) coords.push_back(kv.second.block_coords);
+  shell.GetBlkSparDataTen().DataBlksInsert(coords);
+  return shell;
+}
+
 namespace detail {
 
 inline bool IsIdentityOrder(const std::vector<size_t> &first, const std::vector<size_t> &second) {

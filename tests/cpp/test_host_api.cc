@@ -316,6 +316,28 @@ void TestContractCost(const Fixture &f) {
   zb.Random(U1QN(-1));
   CheckContractCost(za, zb, {2, 1}, {1, 0}, true, true, "cost complex, both permuted");
 
+  // composition over a layout shell (test_tensor_op_cost.cc:44-80 pattern): the
+  // shell keeps the layout, holds no data, and the next estimate on it equals
+  // the estimate on the real contraction result
+  {
+    const auto c1 = EstimateContractCost(a, b, {{2}, {0}});
+    const auto shell = MakeTensorShellFromLayout<QLTEN_Double>(c1.output_layout);
+    CHECK(MakeBlockSparseTensorLayout(shell) == c1.output_layout, "shell: layout round trip");
+    CHECK(!shell.HasActualData(), "shell: no raw data");
+    DTen real_c;
+    Contract(&a, &b, {{2}, {0}}, &real_c);
+    CHECK(real_c.HasActualData(), "shell: the contraction result holds data");
+    DTen e({f.idx_in_s, f.idx_out_s});
+    e.Random(U1QN(0));
+    const auto on_shell = EstimateContractCost(shell, e, {{3}, {0}});
+    const auto on_real = EstimateContractCost(real_c, e, {{3}, {0}});
+    CHECK(on_shell.flops == on_real.flops && on_shell.flops > 0 && on_shell.read_bytes == on_real.read_bytes &&
+              on_shell.write_bytes == on_real.write_bytes && on_shell.output_layout == on_real.output_layout,
+          "shell: composed estimate");
+    CHECK(MakeTensorShellFromLayout<QLTEN_Double>(BlockSparseTensorLayout<U1QN>()).IsDefault(),
+          "shell: default layout gives a default tensor");
+  }
+
   // errors: index mismatch, default operand
   bool threw = false;
   try {
