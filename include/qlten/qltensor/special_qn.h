@@ -1,8 +1,10 @@
-// Quantum-number types of the contraction path: U(1) and U(1)xU(1).
-// API as the reference's special_qn::U1QN (qltensor/special_qn/u1qn.h:27-169)
-// and special_qn::U1U1QN (qltensor/special_qn/u1u1qn.h:27-189).  Equality is by
-// hash, as in the reference; the hash maps are kept identical so that Index
-// equality (hash based) agrees with the reference's.
+// Bosonic quantum-number types of the contraction path: U(1), U(1)xU(1),
+// Z_n and the trivial representation.  API as the reference's
+// special_qn::U1QN (qltensor/special_qn/u1qn.h:27-169), U1U1QN
+// (u1u1qn.h:27-189), ZnQN / Z2QN (znqn.h:27-176) and TrivialRepQN
+// (trivial_rep_qn.h:27-120).  Equality is by hash, as in the reference; the
+// hash maps are kept identical so that Index equality (hash based) agrees with
+// the reference's.
 #ifndef QLTEN_AMD_QLTENSOR_SPECIAL_QN_H
 #define QLTEN_AMD_QLTENSOR_SPECIAL_QN_H
 
@@ -88,6 +90,87 @@ class U1U1QN {
   size_t hash_;
 };
 
+// Z_n charge: addition modulo n.  As in the reference, the constructor keeps
+// the value it is given, operator+ reduces with %, operator+= subtracts n
+// once, and the negation of v != 0 is n - v.
+template <size_t n>
+class ZnQN {
+ public:
+  ZnQN() : ZnQN(0) {}
+  ZnQN(const int val) : val_(val), hash_(CalcHash(val)) {}  // NOLINT: implicit like the reference
+
+  ZnQN operator-() const { return ZnQN(val_ == 0 ? 0 : static_cast<int>(n) - val_); }
+  ZnQN operator+(const ZnQN &rhs) const { return ZnQN((val_ + rhs.val_) % static_cast<int>(n)); }
+  ZnQN operator-(const ZnQN &rhs) const { return *this + (-rhs); }
+  ZnQN &operator+=(const ZnQN &rhs) {
+    val_ += rhs.val_;
+    if (val_ >= static_cast<int>(n)) val_ -= static_cast<int>(n);
+    hash_ = CalcHash(val_);
+    return *this;
+  }
+  bool operator==(const ZnQN &rhs) const { return hash_ == rhs.hash_; }
+  bool operator!=(const ZnQN &rhs) const { return !(*this == rhs); }
+
+  size_t dim() const { return 1; }
+  size_t Hash() const { return hash_; }
+  int GetVal() const { return val_; }
+  static ZnQN Zero() { return ZnQN(0); }
+
+  // Stream format (znqn.h:134-145): the value and the stored hash, one per line.
+  void StreamRead(std::istream &is) { is >> val_ >> hash_; }
+  void StreamWrite(std::ostream &os) const { os << val_ << "\n" << hash_ << "\n"; }
+
+ private:
+  // the vector hash of one item whose hash is the value (znqn.h:158-167)
+  static size_t CalcHash(int v) {
+    constexpr size_t kP1 = 11400714785074694791ull, kP2 = 14029467366897019727ull,
+                     kP5 = 2870177450012600261ull;
+    size_t h = kP5 + static_cast<size_t>(v) * kP2;
+    h = (h << 31) | (h >> 33);
+    h *= kP1;
+    return h + (size_t(1) ^ kP5);
+  }
+  int val_;
+  size_t hash_;
+};
+
+using Z2QN = ZnQN<2>;
+
+// The trivial representation: one charge, hash 0, nothing in the stream.
+class TrivialRepQN {
+ public:
+  TrivialRepQN operator-() const { return {}; }
+  TrivialRepQN &operator+=(const TrivialRepQN &) { return *this; }
+  TrivialRepQN operator+(const TrivialRepQN &) const { return {}; }
+  TrivialRepQN operator-(const TrivialRepQN &) const { return {}; }
+  bool operator==(const TrivialRepQN &) const { return true; }
+  bool operator!=(const TrivialRepQN &) const { return false; }
+  static size_t dim() { return 1; }
+  static size_t Hash() { return 0; }
+  static TrivialRepQN Zero() { return {}; }
+  void StreamRead(std::istream &) {}
+  void StreamWrite(std::ostream &) const {}
+};
+
+template <size_t n>
+inline std::istream &operator>>(std::istream &is, ZnQN<n> &qn) {
+  qn.StreamRead(is);
+  return is;
+}
+template <size_t n>
+inline std::ostream &operator<<(std::ostream &os, const ZnQN<n> &qn) {
+  qn.StreamWrite(os);
+  return os;
+}
+inline std::istream &operator>>(std::istream &is, TrivialRepQN &qn) {
+  qn.StreamRead(is);
+  return is;
+}
+inline std::ostream &operator<<(std::ostream &os, const TrivialRepQN &qn) {
+  qn.StreamWrite(os);
+  return os;
+}
+
 inline std::istream &operator>>(std::istream &is, U1QN &qn) {
   qn.StreamRead(is);
   return is;
@@ -107,8 +190,11 @@ inline std::ostream &operator<<(std::ostream &os, const U1U1QN &qn) {
 
 }  // namespace special_qn
 
+using special_qn::TrivialRepQN;
 using special_qn::U1QN;
 using special_qn::U1U1QN;
+using special_qn::Z2QN;
+using special_qn::ZnQN;
 
 }  // namespace qlten
 

@@ -40,8 +40,8 @@ static int g_checks = 0;
   } while (0)
 
 // Dense host copy of a tensor (row-major over the full index dimensions).
-template <typename T>
-std::vector<T> Densify(const QLTensor<T, U1QN> &t) {
+template <typename T, typename Q>
+std::vector<T> Densify(const QLTensor<T, Q> &t) {
   size_t total = 1;
   for (size_t d : t.GetShape()) total *= d;
   std::vector<T> out(total, T(0));
@@ -153,10 +153,10 @@ struct Fixture {
   }
 };
 
-template <typename T>
-void CheckContract(const QLTensor<T, U1QN> &a, const QLTensor<T, U1QN> &b, const std::vector<size_t> &a_ax,
+template <typename T, typename Q>
+void CheckContract(const QLTensor<T, Q> &a, const QLTensor<T, Q> &b, const std::vector<size_t> &a_ax,
                    const std::vector<size_t> &b_ax, const std::string &name) {
-  QLTensor<T, U1QN> c;
+  QLTensor<T, Q> c;
   Contract(&a, &b, {a_ax, b_ax}, &c);
   std::vector<size_t> a_sv, b_sv;
   for (size_t i = 0; i < a.Rank(); ++i)
@@ -665,6 +665,38 @@ int main() {
       d3.Random(qnp1);
       out2 << d3;
     }
+  }
+
+  // ---- Z2 and trivial quantum numbers (special_qn/znqn.h, trivial_rep_qn.h) ----
+  {
+    using Z2Idx = Index<Z2QN>;
+    const Z2QN z0(0), z1(1);
+    CHECK(z1 + z1 == z0 && -z1 == z1 && -z0 == z0 && z1 - z1 == z0, "Z2 arithmetic");
+    Z2QN acc = z1;
+    acc += z1;
+    CHECK(acc == z0 && acc.GetVal() == 0, "Z2 +=");
+    CHECK(ZnQN<3>(1) + ZnQN<3>(2) == ZnQN<3>(0) && -ZnQN<3>(1) == ZnQN<3>(2), "Z3 arithmetic");
+    CHECK(z0.Hash() != z1.Hash(), "Z2 hashes differ");
+    const Z2Idx zo({QNSector<Z2QN>(z0, 2), QNSector<Z2QN>(z1, 3)}, OUT), zi = InverseIndex(zo);
+    const Z2Idx wo({QNSector<Z2QN>(z0, 4), QNSector<Z2QN>(z1, 1)}, OUT), wi = InverseIndex(wo);
+    QLTensor<QLTEN_Double, Z2QN> za({zi, wo, zo}), zb({zi, wi, zo});
+    za.Random(z0);
+    zb.Random(z1);
+    CheckContract(za, zb, {2}, {0}, "Z2 rank3 tail-head");
+    CheckContract(za, zb, {1, 2}, {1, 0}, "Z2 rank3 two axes");
+    QLTensor<QLTEN_Complex, Z2QN> zc({zi, wo}), zd({wi, zo});
+    zc.Random(z1);
+    zd.Random(z1);
+    CheckContract(zc, zd, {1}, {0}, "Z2 complex rank2");
+    CheckFileIO(za, "Z2 tensor");
+
+    using TIdx = Index<TrivialRepQN>;
+    const TIdx to({QNSector<TrivialRepQN>(TrivialRepQN(), 5)}, OUT), ti = InverseIndex(to);
+    const TIdx so({QNSector<TrivialRepQN>(TrivialRepQN(), 3)}, OUT);
+    QLTensor<QLTEN_Double, TrivialRepQN> ta({so, to}), tb({ti, so});
+    ta.Random(TrivialRepQN());
+    tb.Random(TrivialRepQN());
+    CheckContract(ta, tb, {1}, {0}, "trivial QN dense contraction");
   }
 
   // ---- contraction cost estimate (tensor_op_cost.h) ----
