@@ -1,7 +1,8 @@
 // Bosonic quantum-number types of the contraction path: U(1), U(1)xU(1),
-// Z_n and the trivial representation.  API as the reference's
+// Z_n, U(1)xU(1)xZ_n and the trivial representation.  API as the reference's
 // special_qn::U1QN (qltensor/special_qn/u1qn.h:27-169), U1U1QN
-// (u1u1qn.h:27-189), ZnQN / Z2QN (znqn.h:27-176) and TrivialRepQN
+// (u1u1qn.h:27-189), ZnQN / Z2QN (znqn.h:27-176), U1U1ZnQN
+// (u1u1znqn.h:27-227) and TrivialRepQN
 // (trivial_rep_qn.h:27-120).  Equality is by hash, as in the reference; the
 // hash maps are kept identical so that Index equality (hash based) agrees with
 // the reference's.
@@ -100,11 +101,12 @@ class ZnQN {
   ZnQN(const int val) : val_(val), hash_(CalcHash(val)) {}  // NOLINT: implicit like the reference
 
   ZnQN operator-() const { return ZnQN(val_ == 0 ? 0 : static_cast<int>(n) - val_); }
-  ZnQN operator+(const ZnQN &rhs) const { return ZnQN((val_ + rhs.val_) % static_cast<int>(n)); }
+  // the sum taken modulo n as an unsigned number, as in the reference
+  ZnQN operator+(const ZnQN &rhs) const { return ZnQN(static_cast<int>(static_cast<size_t>(val_ + rhs.val_) % n)); }
   ZnQN operator-(const ZnQN &rhs) const { return *this + (-rhs); }
   ZnQN &operator+=(const ZnQN &rhs) {
     val_ += rhs.val_;
-    if (val_ >= static_cast<int>(n)) val_ -= static_cast<int>(n);
+    if (static_cast<size_t>(val_) >= n) val_ -= static_cast<int>(n);
     hash_ = CalcHash(val_);
     return *this;
   }
@@ -136,6 +138,57 @@ class ZnQN {
 
 using Z2QN = ZnQN<2>;
 
+// U(1) x U(1) x Z_n charge (u1u1znqn.h:27-227): both U(1) values add, the
+// Z_n value adds as in ZnQN.
+template <size_t n>
+class U1U1ZnQN {
+ public:
+  U1U1ZnQN() : U1U1ZnQN(0, 0, 0) {}
+  U1U1ZnQN(const int u1, const int u2, const int zn) : u1_{u1, u2}, zn_(zn), hash_(CalcHash(u1, u2, zn)) {}
+
+  U1U1ZnQN operator-() const { return U1U1ZnQN(-u1_[0], -u1_[1], zn_ == 0 ? 0 : static_cast<int>(n) - zn_); }
+  U1U1ZnQN &operator+=(const U1U1ZnQN &rhs) {
+    u1_[0] += rhs.u1_[0];
+    u1_[1] += rhs.u1_[1];
+    zn_ += rhs.zn_;
+    if (static_cast<size_t>(zn_) >= n) zn_ -= static_cast<int>(n);
+    hash_ = CalcHash(u1_[0], u1_[1], zn_);
+    return *this;
+  }
+  U1U1ZnQN operator+(const U1U1ZnQN &rhs) const {
+    U1U1ZnQN r(*this);
+    r += rhs;
+    return r;
+  }
+  U1U1ZnQN operator-(const U1U1ZnQN &rhs) const { return *this + (-rhs); }
+  bool operator==(const U1U1ZnQN &rhs) const { return hash_ == rhs.hash_; }
+  bool operator!=(const U1U1ZnQN &rhs) const { return !(*this == rhs); }
+
+  size_t dim() const { return 1; }
+  size_t Hash() const { return hash_; }
+  int GetVal(size_t i) const { return i < 2 ? u1_[i] : zn_; }
+  static U1U1ZnQN Zero() { return U1U1ZnQN(0, 0, 0); }
+
+  // Stream format (u1u1znqn.h:165-179): the three values and the stored hash.
+  void StreamRead(std::istream &is) { is >> u1_[0] >> u1_[1] >> zn_ >> hash_; }
+  void StreamWrite(std::ostream &os) const {
+    os << u1_[0] << "\n" << u1_[1] << "\n" << zn_ << "\n" << hash_ << "\n";
+  }
+
+ private:
+  // U(1) values offset by 2^20 in bits 0-20 and 21-41, the Z_n value from
+  // bit 42, rotated left by 10 bits (u1u1znqn.h:205-225)
+  static size_t CalcHash(int u1, int u2, int zn) {
+    const size_t seg = size_t(1) << 20;
+    const size_t h = (static_cast<size_t>(u1) + seg) + (static_cast<size_t>(u2) + seg) * (2 * seg) +
+                     static_cast<size_t>(zn) * (4 * seg * seg);
+    return (h << 10) | (h >> 54);
+  }
+  int u1_[2];
+  int zn_;
+  size_t hash_;
+};
+
 // The trivial representation: one charge, hash 0, nothing in the stream.
 class TrivialRepQN {
  public:
@@ -159,6 +212,16 @@ inline std::istream &operator>>(std::istream &is, ZnQN<n> &qn) {
 }
 template <size_t n>
 inline std::ostream &operator<<(std::ostream &os, const ZnQN<n> &qn) {
+  qn.StreamWrite(os);
+  return os;
+}
+template <size_t n>
+inline std::istream &operator>>(std::istream &is, U1U1ZnQN<n> &qn) {
+  qn.StreamRead(is);
+  return is;
+}
+template <size_t n>
+inline std::ostream &operator<<(std::ostream &os, const U1U1ZnQN<n> &qn) {
   qn.StreamWrite(os);
   return os;
 }
@@ -193,6 +256,7 @@ inline std::ostream &operator<<(std::ostream &os, const U1U1QN &qn) {
 using special_qn::TrivialRepQN;
 using special_qn::U1QN;
 using special_qn::U1U1QN;
+using special_qn::U1U1ZnQN;
 using special_qn::Z2QN;
 using special_qn::ZnQN;
 

@@ -690,6 +690,19 @@ int main() {
     CheckContract(zc, zd, {1}, {0}, "Z2 complex rank2");
     CheckFileIO(za, "Z2 tensor");
 
+    using Q3 = U1U1ZnQN<2>;
+    CHECK(Q3().Hash() == (size_t(1) << 30) + (size_t(1) << 51), "U1U1Z2 zero hash (2^20 + 2^41, rotated by 10)");
+    CHECK(Q3(1, -2, 1) + Q3(-1, 2, 1) == Q3() && -Q3(3, 1, 1) == Q3(-3, -1, 1), "U1U1Z2 arithmetic");
+    const Index<Q3> qo({QNSector<Q3>(Q3(0, 0, 0), 2), QNSector<Q3>(Q3(1, 0, 1), 3), QNSector<Q3>(Q3(0, 1, 1), 2)},
+                       OUT),
+        qi = InverseIndex(qo);
+    QLTensor<QLTEN_Double, Q3> qa({qi, qo, qo}), qb({qi, qi, qo});
+    qa.Random(Q3(1, 0, 1));
+    qb.Random(Q3(0, 0, 0));
+    CheckContract(qa, qb, {2}, {0}, "U1U1Z2 rank3 tail-head");
+    CheckContract(qa, qb, {1, 2}, {1, 0}, "U1U1Z2 rank3 two axes");
+    CheckFileIO(qa, "U1U1Z2 tensor");
+
     using TIdx = Index<TrivialRepQN>;
     const TIdx to({QNSector<TrivialRepQN>(TrivialRepQN(), 5)}, OUT), ti = InverseIndex(to);
     const TIdx so({QNSector<TrivialRepQN>(TrivialRepQN(), 3)}, OUT);
