@@ -933,6 +933,10 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
         b0[s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][comp][s2][wn + lrow][lk]);
         b1[s2] = *reinterpret_cast<const v8bf *>(&Bs[buf][1 - comp][s2][wn + lrow][lk]);
       }
+      // a raised wave priority while the MFMAs issue: this wave wins the
+      // issue arbitration against the fragment reads of the other wave on
+      // its SIMD (the kernel runs about 1 % faster)
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int sa = 0; sa < S; ++sa) {
         if (sa > 0) {
@@ -947,6 +951,7 @@ __global__ __launch_bounds__(512) void k_split_gemm_c2(
           g[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai, b1[sb], g[q], 0, 0, 0);
         }
       }
+      __builtin_amdgcn_s_setprio(0);
       const int kk = (kc + 1) * KC;                // k since the task start
       const bool last = (kc == nkc - 1);
       if (last || kk % 32 == 0) {
