@@ -18,8 +18,9 @@ Reference steps restated:
 Pinning: the known-answer singular values of [[1, 2], [3, 4]]
 (tests/package_consumer/smoke_gpu.cu:100-117), the singular values of the
 densified tensor's matricisation (the union over block matrices), U S Vt
-reconstruction, and the truncation weight closed form, in
-tests/test_decomp_oracle.py.
+reconstruction, the truncation weight closed form, and for the QR: Q R equal
+to the assembled block matrix, orthonormal Q, triangular R, every block at
+its sector offsets, in tests/test_decomp_oracle.py.
 """
 from __future__ import annotations
 

@@ -26,8 +26,13 @@ qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode) {
   QLT_CHECK_ARG(valid_mode(mode), "qlt_plan_prepare: bad mode");
   if (mode == QLT_MODE_NATIVE || plan->n_tiles == 0) return QLT_OK;
   if (!split_profitable(plan, dtype, mode)) return QLT_OK;
+  if (plan->split_dtype == dtype && plan->split_mode == mode) return QLT_OK;
   size_t need = 0;
   QLT_TRY(split_workspace_bytes(plan, dtype, mode, &need));
+  // An earlier split execute of this plan may still be reading the workspace
+  // on a non-blocking stream: wait for the device before it is freed or its
+  // records are rewritten.
+  if (plan->d_work != nullptr) QLT_HIP(hipDeviceSynchronize());
   if (need > plan->work_bytes) {
     if (plan->d_work != nullptr) QLT_HIP(hipFree(plan->d_work));
     plan->d_work = nullptr;
@@ -37,7 +42,7 @@ qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode) {
     QLT_HIP(hipMalloc(&plan->d_work, need));
     plan->work_bytes = need;
   }
-  if (plan->split_dtype != dtype || plan->split_mode != mode) QLT_TRY(split_prepare_metadata(plan, dtype, mode));
+  QLT_TRY(split_prepare_metadata(plan, dtype, mode));
   return QLT_OK;
 }
 

@@ -290,12 +290,14 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
     const int grow = it.r0 + r;
     if (I8 && grow < pm.r_pad) {
       int8_t *planes = static_cast<int8_t *>(planes_v);
-      // x 2^(46 - e) is exact (a power of two) and below 2^46 in magnitude.
+      // x 2^(46 - e) is exact (a power of two) and below 2^46 in magnitude;
+      // it is applied by ldexp on each element, since 2^(46 - e) alone
+      // overflows for rows whose maximum is below 2^-977.
       // Digits from the least significant up, all exact in FP64: q is the
       // integer nearest to Y / 255 (the product's error, below 2^-14, stays
       // far inside the 1/510 distance of Y / 255 from a half integer), so
       // Y - 255 q is the balanced digit in [-127, 127].
-      const double sc = ldexp(1.0, 46 - s_exp[r]);
+      const int sc = 46 - s_exp[r];
 #pragma unroll
       for (int part = 0; part < PARTS; ++part) {
         uint32_t wd[S8][RUN / 4];
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
           for (int v = 0; v < RUN / 4; ++v) wd[sl][v] = 0u;
 #pragma unroll
         for (int u = 0; u < RUN; ++u) {
-          double Y = rint(s_tile[part][r][pr + u] * sc);
+          double Y = rint(ldexp(s_tile[part][r][pr + u], sc));
 #pragma unroll
           for (int sl = S8 - 1; sl >= 0; --sl) {
             const double q = rint(Y * (1.0 / 255.0));
@@ -332,13 +334,13 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
       }
     } else if (!I8 && grow < pm.r_pad) {
       uint16_t *planes = static_cast<uint16_t *>(planes_v);
-      const double inv = ldexp(1.0, -s_exp[r]);
+      const int inv = -s_exp[r];   // ldexp per element: 2^-e overflows for subnormal maxima
 #pragma unroll
       for (int part = 0; part < PARTS; ++part) {
         uint16_t out[S][RUN];
 #pragma unroll
         for (int u = 0; u < RUN; ++u) {
-          double y = s_tile[part][r][pr + u] * inv;   // exact (power of two)
+          double y = ldexp(s_tile[part][r][pr + u], inv);   // exact (power of two)
 #pragma unroll
           for (int s = 0; s < S; ++s) {
             const double scale = ldexp(1.0, 8 + 9 * s);
@@ -471,13 +473,13 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
   const int nkc = pm.k_pad / KCF;
   if constexpr (I8) {
     int8_t *planes = static_cast<int8_t *>(planes_v);
-    const double sc = ldexp(1.0, 46 - ex);
+    const int sc = 46 - ex;   // ldexp per element: 2^(46 - e) overflows below 2^-977
 #pragma unroll
     for (int part = 0; part < PARTS; ++part) {
       uint32_t wd[S8][2] = {};
 #pragma unroll
       for (int u = 0; u < RUN; ++u) {
-        double Y = rint(x[part][u] * sc);
+        double Y = rint(ldexp(x[part][u], sc));
 #pragma unroll
         for (int sl = S8 - 1; sl >= 0; --sl) {
           const double q = rint(Y * (1.0 / 255.0));
@@ -495,13 +497,13 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
     }
   } else {
     uint16_t *planes = static_cast<uint16_t *>(planes_v);
-    const double inv = ldexp(1.0, -ex);
+    const int inv = -ex;   // ldexp per element: 2^-e overflows for subnormal maxima
 #pragma unroll
     for (int part = 0; part < PARTS; ++part) {
       uint16_t out[S][RUN];
 #pragma unroll
       for (int u = 0; u < RUN; ++u) {
-        double y = x[part][u] * inv;                 // exact (power of two)
+        double y = ldexp(x[part][u], inv);           // exact (power of two)
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           const double scale = ldexp(1.0, 8 + 9 * s);
@@ -1024,8 +1026,10 @@ __device__ __forceinline__ v4i neg_digits(v4i x) {
 // consecutive rows cover every bank once.
 __device__ __forceinline__ int i8_half(int row, int half) { return 16 * (half ^ ((row >> 2) & 1)); }
 
-// Recombines the six int32 digit levels of accumulator element r in FP64 and
-// adds 255^5 2^(e - 92) times the result to acc.
+// Recombines the six int32 digit levels of accumulator element r in FP64.
+// The caller adds ldexp(sum * 255^5, e - 92) to its FP64 accumulator: the
+// power of two comes last, so a result in the subnormal range is rounded once
+// instead of passing through a subnormal intermediate.
 __device__ __forceinline__ double i8_level_sum(const v16i (&g)[S8], int r) {
   double sum = static_cast<double>(g[0][r]);
 #pragma unroll
@@ -1137,7 +1141,7 @@ __global__ __launch_bounds__(256, 2) void k_split_i8_f64(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int row = wm + 8 * (r >> 2) + 4 * lh + (r & 3);
-          acc64[r] = fma(ldexp(i8_level_sum(g, r), s_ea[buf][row] + eb - 92), k255p5, acc64[r]);
+          acc64[r] += ldexp(i8_level_sum(g, r) * k255p5, s_ea[buf][row] + eb - 92);
         }
 #pragma unroll
         for (int l = 0; l < S8; ++l) g[l] = v16i{};
@@ -1306,7 +1310,7 @@ __global__ __launch_bounds__(512) void k_split_i8_c2(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int row = wm + 8 * (r >> 2) + 4 * lh + (r & 3);
-          acc64[r] = fma(ldexp(i8_level_sum(g, r), s_ea[buf][row] + eb - 92), k255p5, acc64[r]);
+          acc64[r] += ldexp(i8_level_sum(g, r) * k255p5, s_ea[buf][row] + eb - 92);
         }
 #pragma unroll
         for (int l = 0; l < S8; ++l) g[l] = v16i{};

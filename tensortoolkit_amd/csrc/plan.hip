@@ -1120,6 +1120,14 @@ qlt_status qlt_plan_set_c_offsets(qlt_plan *plan, const int64_t *c_offsets) {
   const int64_t n = static_cast<int64_t>(plan->h_segs.size());
   if (n == 0) return QLT_OK;
   QLT_CHECK_ARG(c_offsets != nullptr, "qlt_plan_set_c_offsets: null offsets");
+  bool same = true;
+  for (int64_t o = 0; o < n && same; ++o) same = plan->h_segs[o].c_off == c_offsets[o];
+  if (same) return QLT_OK;
+  // The kernels of an earlier execute of this plan may still be running on a
+  // non-blocking stream and read the segment records (and the split tile
+  // records, which carry copies of the output offsets): wait for the whole
+  // device before the records change.
+  QLT_HIP(hipDeviceSynchronize());
   if (plan->c_is_scalar) {
     plan->h_segs[0].c_off = c_offsets[0];
     for (qlt_task &t : plan->tasks) t.c_data_offset = c_offsets[0];
@@ -1136,6 +1144,9 @@ qlt_status qlt_plan_set_c_offsets(qlt_plan *plan, const int64_t *c_offsets) {
     }
   }
   QLT_HIP(hipMemcpy(plan->d_segs, plan->h_segs.data(), sizeof(Seg) * n, hipMemcpyHostToDevice));
+  // the split metadata holds the old offsets: the next split execute rewrites it
+  plan->split_dtype = -1;
+  plan->split_mode = -1;
   return QLT_OK;
 }
 

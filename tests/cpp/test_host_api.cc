@@ -480,6 +480,71 @@ int main() {
     CHECK(threw, "aliasing throws");
   }
 
+  // ---- accumulate into an output whose layout is not the plan's own, in every
+  // arithmetic mode (contract_contiguous_axes.h:389-470, 953-1041): C holds
+  // blocks of another divergence between the touched ones, so the touched
+  // blocks sit at offsets other than the plan's; untouched blocks are scaled
+  // by beta; a missing touched block expands the topology; a plain Contract
+  // on the same cached plan afterwards writes at the plan's own offsets ----
+  {
+    QNSectorVec<U1QN> big = {QNSctT(U1QN(-1), 40), QNSctT(U1QN(0), 50), QNSctT(U1QN(1), 36)};
+    const U1Idx bi(big, IN), bo(big, OUT);
+    auto run = [&](auto tag, hp_numeric::ContractPrecision prec, auto alpha, auto beta, const std::string &name) {
+      using T = decltype(tag);
+      using Ten = QLTensor<T, U1QN>;
+      Ten a({bi, bo}), b({bi, bo}), x0({bi, bo}), x1({bi, bo});
+      a.Random(U1QN(0));
+      b.Random(U1QN(0));
+      x0.Random(U1QN(0));
+      x1.Random(U1QN(1));
+      if (prec != hp_numeric::ContractPrecision::kNativeFP64) QltCheck(qlt_set_split_min_mean_k(0.0));
+      hp_numeric::SetContractPrecision(prec);
+      const auto ab = DenseContract(Densify(a), a.GetShape(), Densify(b), b.GetShape(), {1}, {0}, {0}, {1});
+      auto expect_of = [&](const Ten &init) {
+        auto e = Densify(init);
+        for (size_t i = 0; i < e.size(); ++i) e[i] = beta * e[i] + alpha * ab[i];
+        return e;
+      };
+      // interleaved layout: blocks of divergence 0 and 1
+      Ten c = x0 + x1;
+      CHECK(c.GetBlkSparDataTen().GetBlkIdxDataBlkMap().size() == 3 + 2, name + ": interleaved layout");
+      const auto e1 = expect_of(c);
+      ContiguousContractStats st;
+      ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, T(alpha), T(beta), c, &st);
+      double err = RelDiff(Densify(c), e1);
+      CHECK(err < 1e-12, name + ": accumulate into interleaved layout, rel err " + std::to_string(err));
+      CHECK(st.output_topology_expansions == 0, name + ": no expansion needed");
+      // a second call on the same target reuses the redirected plan
+      const auto e2 = expect_of(c);
+      ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, T(alpha), T(beta), c);
+      err = RelDiff(Densify(c), e2);
+      CHECK(err < 1e-12, name + ": second accumulate, rel err " + std::to_string(err));
+      // expansion: only the untouched blocks exist
+      Ten g = x1;
+      CHECK(!TryContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, T(alpha), T(beta), g),
+            name + ": Try refuses expansion");
+      const auto e3 = expect_of(g);
+      ContractTailHeadContiguousAccumulate(a, b, 1, 0, 1, T(alpha), T(beta), g, &st);
+      err = RelDiff(Densify(g), e3);
+      CHECK(err < 1e-12, name + ": accumulate with expansion, rel err " + std::to_string(err));
+      CHECK(st.output_topology_expansions == 1, name + ": one expansion");
+      // the cached plan, restored, serves a plain Contract
+      Ten p;
+      Contract(&a, &b, {{1}, {0}}, &p);
+      err = RelDiff(Densify(p), ab);
+      CHECK(err < 1e-12, name + ": plain Contract after accumulate, rel err " + std::to_string(err));
+      hp_numeric::SetContractPrecision(hp_numeric::ContractPrecision::kNativeFP64);
+      QltCheck(qlt_set_split_min_mean_k(48.0));
+    };
+    for (auto prec : {hp_numeric::ContractPrecision::kNativeFP64, hp_numeric::ContractPrecision::kSplitBF16,
+                      hp_numeric::ContractPrecision::kSplitInt8}) {
+      const std::string m = "accumulate mode " + std::to_string(static_cast<int>(prec));
+      run(QLTEN_Double(0), prec, 1.5, -0.5, m + " real");
+      run(QLTEN_Complex(0), prec, QLTEN_Complex(0.5, -1.5), QLTEN_Complex(2.0, 0.25), m + " complex");
+      run(QLTEN_Complex(0), prec, QLTEN_Complex(2.0, 0.0), QLTEN_Complex(0.5, 0.0), m + " complex, real scalars");
+    }
+  }
+
   // ---- plan cache reuse on a repeated pattern ----
   {
     DTen a({f.idx_in_l, f.idx_out_l});

@@ -73,8 +73,27 @@ def test_truncation_weight_matches_discarded_norm():
     assert abs(np.linalg.norm(d - M) ** 2 / np.linalg.norm(d) ** 2 - err) < 1e-12
 
 
-def test_qr_blocks_reconstruct():
-    t = rand_tensor(3, "c128")
-    for lf, Q, R in OD.qr_blocks(t, 2):
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_qr_blocks_reconstruct(dtype):
+    """Q R equals the assembled block matrix of its left flux, Q has
+    orthonormal columns and R is upper triangular; the block matrices hold
+    every block of the tensor once (their squared norms add up to the
+    tensor's), so a QR of the wrong matrix, or a dropped block, fails."""
+    t = rand_tensor(3, dtype)
+    groups = OD.group(t, 2)
+    res = OD.qr_blocks(t, 2)
+    assert [lf for lf, _, _ in res] == [g[0] for g in groups]
+    total = 0.0
+    for (lf, Q, R), (_, rows, cols, blks) in zip(res, groups):
+        A, roff, coff = OD.dense_matrix(t, rows, cols, blks)
+        assert Q.shape == (A.shape[0], min(A.shape)) and R.shape == (min(A.shape), A.shape[1])
+        np.testing.assert_allclose(Q @ R, A, rtol=0, atol=1e-13 * max(1.0, np.abs(A).max()))
         assert np.allclose(Q.conj().T @ Q, np.eye(Q.shape[1]), atol=1e-13)
-        assert np.allclose(np.tril(R, -1), 0)
+        assert np.all(np.tril(R, -1) == 0)
+        total += np.linalg.norm(A) ** 2
+        # each block sits at its row / column sector offsets
+        for lp, rp, b in blks:
+            m, n = rows[lp], cols[rp]
+            np.testing.assert_array_equal(A[roff[lp]:roff[lp] + m, coff[rp]:coff[rp] + n],
+                                          t.block(b).reshape(m, n))
+    assert abs(total - np.linalg.norm(t.data) ** 2) <= 1e-12 * total

@@ -152,6 +152,41 @@ def test_split_wide_dynamic_range(forced_split, split_mode, dtype):
     assert rel_frobenius(c.data, exp) <= TOL
 
 
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_tiny_and_subnormal_rows(forced_split, split_mode, dtype):
+    """Rows of A whose maximum is 2^-1000 (normal) or subnormal (2^-1050,
+    2^-1070), and columns of B scaled to 2^-1030: the power-of-two scale of
+    such a row (2^(46 - e) for int8 digits, 2^-e for BF16 slices) overflows
+    FP64 on its own, so the pack must apply it to each element."""
+    ix_m = Index(((0,),), (40,), IN)
+    ix_k = Index(((0,),), (90,), OUT)
+    ix_n = Index(((0,),), (50,), OUT)
+    a = BlockSparseTensor.allowed_blocks([ix_m, ix_k], (0,), dtype).fill_uniform(33)
+    b = BlockSparseTensor.allowed_blocks([ix_k.inverse(), ix_n], (0,), dtype).fill_uniform(34)
+    rs = np.ones(40)
+    rs[0:8] = 2.0 ** -1000
+    rs[8:12] = 2.0 ** -1050
+    rs[12] = 2.0 ** -1070
+    cs = np.ones(50)
+    cs[45:] = 2.0 ** -1030
+    a.data = (a.data.reshape(40, 90) * rs[:, None]).ravel()
+    b.data = (b.data.reshape(90, 50) * cs[None, :]).ravel()
+    plan, c = gpu_contract(a, b, [1], [0], mode=split_mode)
+    exp = O.OraclePlan(a, b, [1], [0]).execute()
+    got = c.data.reshape(40, 50)
+    ref = exp.reshape(40, 50)
+    assert np.all(np.isfinite(got))
+    # rows whose products stay normal: relative accuracy of the row
+    for i in list(range(8)) + list(range(13, 40)):
+        assert rel_frobenius(got[i, :45], ref[i, :45]) <= 1e-11, ("row", i)
+    # subnormal results: both sides round every product to the 2^-1074 grid
+    # (the oracle once per product, 90 products per entry)
+    tiny = 2.0 ** -1074
+    assert np.max(np.abs(got[8:13] - ref[8:13])) <= 256 * tiny
+    assert np.max(np.abs(got[:, 45:] - ref[:, 45:])) <= 1e-11 * np.max(np.abs(ref[:, 45:])) + 256 * tiny
+    assert rel_frobenius(c.data, exp) <= TOL
+
+
 def test_split_zero_blocks_and_rows(forced_split, split_mode):
     ix = Index(((0,),), (70,), OUT)
     a = BlockSparseTensor.allowed_blocks([ix.inverse(), ix], (0,)).fill_uniform(41)
