@@ -171,6 +171,56 @@ void CheckContract(const QLTensor<T, Q> &a, const QLTensor<T, Q> &b, const std::
   CHECK(c.Rank() == a_sv.size() + b_sv.size(), name + ": rank");
 }
 
+// Contiguous-axes contraction in the four (ASide, BSide) forms and the four
+// legacy bool forms (RunTestTenMatBasedCtrct, test_ten_ctrct.cc:638-735).
+// The benchmark follows the reference test: A is viewed with its axes after
+// the contracted run first, then the axes up to the run's end (trans_axes_a,
+// :673-680), which puts A's saved axes in the cyclic order
+// back + 1, ..., rank - 1, 0, ..., front - 1; likewise B.  The dense
+// contraction with those saved orders is computed on the host.
+template <typename T>
+void CheckMatBasedContract(const QLTensor<T, U1QN> &ta, const std::vector<size_t> &a_ax,
+                           const QLTensor<T, U1QN> &tb, const std::vector<size_t> &b_ax, const std::string &name) {
+  auto cyclic_saved = [](size_t rank, const std::vector<size_t> &ax) {
+    std::vector<size_t> sv;
+    for (size_t i = ax.back() + 1; i < rank; ++i) sv.push_back(i);
+    for (size_t i = 0; i < ax.front(); ++i) sv.push_back(i);
+    return sv;
+  };
+  const auto a_sv = cyclic_saved(ta.Rank(), a_ax), b_sv = cyclic_saved(tb.Rank(), b_ax);
+  const auto ref = DenseContract(Densify(ta), ta.GetShape(), Densify(tb), tb.GetShape(), a_ax, b_ax, a_sv, b_sv);
+  IndexVec<U1QN> ref_idx;
+  for (size_t x : a_sv) ref_idx.push_back(ta.GetIndex(x));
+  for (size_t x : b_sv) ref_idx.push_back(tb.GetIndex(x));
+  const size_t a0 = a_ax[0], b0 = b_ax[0], n = a_ax.size();
+  std::vector<std::pair<std::string, QLTensor<T, U1QN>>> res(8);
+  Contract<T, U1QN, true, true>(ta, tb, a0, b0, n, res[0].second);
+  Contract<T, U1QN, true, false>(ta, tb, a0, b0, n, res[1].second);
+  Contract<T, U1QN, false, true>(ta, tb, a0, b0, n, res[2].second);
+  Contract<T, U1QN, false, false>(ta, tb, a0, b0, n, res[3].second);
+  ContractContiguousAxes<T, U1QN, CtrctSide::Tail, CtrctSide::Head>(ta, tb, a0, b0, n, res[4].second);
+  ContractContiguousAxes<T, U1QN, CtrctSide::Tail, CtrctSide::Tail>(ta, tb, a0, b0, n, res[5].second);
+  ContractContiguousAxes<T, U1QN, CtrctSide::Head, CtrctSide::Head>(ta, tb, a0, b0, n, res[6].second);
+  ContractContiguousAxes<T, U1QN, CtrctSide::Head, CtrctSide::Tail>(ta, tb, a0, b0, n, res[7].second);
+  const char *forms[8] = {"<true,true>", "<true,false>", "<false,true>", "<false,false>",
+                          "<Tail,Head>", "<Tail,Tail>", "<Head,Head>", "<Head,Tail>"};
+  for (size_t i = 0; i < res.size(); ++i) {
+    const auto &c = res[i].second;
+    const std::string nm = name + " " + forms[i];
+    CHECK(c.GetIndexes() == ref_idx, nm + ": result indexes in cyclic saved order");
+    const auto got = Densify(c);
+    CHECK(got.size() == ref.size(), nm + ": size");
+    if (got.size() == ref.size()) {
+      const double err = RelDiff(got, ref);
+      CHECK(err < 1e-13, nm + ": rel err " + std::to_string(err));
+    }
+    if (i > 0) {
+      CHECK(c.GetBlkSparDataTen().GetActualRawDataSize() == res[0].second.GetBlkSparDataTen().GetActualRawDataSize(),
+            nm + ": same block shell");
+    }
+  }
+}
+
 // .qlten round trip through a file (RunTestQLTensorFileIOCase,
 // tests/test_qltensor/test_qltensor.cc:1029-1041).
 template <typename TenT>
@@ -443,6 +493,44 @@ int main() {
     DTen c_legacy;
     Contract<QLTEN_Double, U1QN, true, true>(a, b, 2, 0, 2, c_legacy);
     CHECK(RelDiff(Densify(c_legacy), Densify(c_gen)) < 1e-13, "legacy bool Contract");
+  }
+
+  // ---- the reference's MatBasedContract cases (test_ten_ctrct.cc:737-859):
+  // axes {2}/{0} (cyclic = ascending), {1}/{0} (A saved (2, 0): cyclic order
+  // differs from ascending), {1,2}/{0,1}, {0,1,2}/{0,1,2} (scalar), every
+  // divergence pair the reference uses; real and complex ----
+  {
+    const U1QN q0(0), qp(1), qm(-1);
+    auto cases = [&](auto tag, const std::string &ty) {
+      using T = decltype(tag);
+      using Ten = QLTensor<T, U1QN>;
+      Ten s3({f.idx_in_s, f.idx_out_s, f.idx_out_s});
+      Ten s3b({f.idx_in_s, f.idx_out_s, f.idx_out_s});
+      Ten s3c({f.idx_in_s, f.idx_in_s, f.idx_out_s});
+      Ten s3d({f.idx_out_s, f.idx_in_s, f.idx_in_s});
+      const std::vector<std::pair<U1QN, U1QN>> divs = {{q0, q0}, {qp, q0}, {qp, qm}};
+      for (const auto &dv : divs) {
+        const std::string d = ty + " div (" + std::to_string(dv.first.GetVal()) + "," +
+                              std::to_string(dv.second.GetVal()) + ")";
+        s3.Random(dv.first);
+        s3b.Random(dv.second);
+        CheckMatBasedContract(s3, {2}, s3b, {0}, d + " {2}/{0}");
+        CheckMatBasedContract(s3, {1}, s3b, {0}, d + " {1}/{0}");
+        s3c.Random(dv.second);
+        CheckMatBasedContract(s3, {1, 2}, s3c, {0, 1}, d + " {1,2}/{0,1}");
+        s3d.Random(dv.second);
+        CheckMatBasedContract(s3, {0, 1, 2}, s3d, {0, 1, 2}, d + " {0,1,2}/{0,1,2}");
+      }
+      // rank 4 with the run in the middle of A and at the end of B: A saved
+      // (3, 0), B saved (0, 1)
+      Ten a4({f.idx_in_s, f.idx_out_l, f.idx_out_s, f.idx_in_l});
+      Ten b4({f.idx_in_s, f.idx_out_l, f.idx_in_l, f.idx_in_s});
+      a4.Random(q0);
+      b4.Random(qp);
+      CheckMatBasedContract(a4, {1, 2}, b4, {2, 3}, ty + " rank4 {1,2}/{2,3}");
+    };
+    cases(QLTEN_Double(0), "real");
+    cases(QLTEN_Complex(0), "complex");
   }
 
   // ---- accumulate: C = beta C + alpha A.B (contract_contiguous_axes.h:664-782) ----

@@ -160,3 +160,23 @@ def test_scalar_output_breaks_after_first_match():
     assert (t["c_data_offset"] == 0).all()
     assert t["beta"][0] == 0.0 and (t["beta"][1:] == 1.0).all()
     assert len(set(t["a_blk_idx"])) == len(t)
+
+
+@pytest.mark.parametrize("shape", __import__("tests.contiguous_cases", fromlist=["SHAPES"]).SHAPES)
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_oracle_cyclic_saved_axes_equal_dense(shape, dtype):
+    """The oracle under the contiguous path's explicit saved axes (cyclic from
+    the end of the contracted run, contract_contiguous_axes.h:342-353)
+    equals the dense contraction with its free axes in that order: a plan
+    that ignored the given order (ascending saved axes) fails the cases whose
+    cyclic order differs."""
+    from tests.contiguous_cases import contiguous_case, dense_reference
+    rng = np.random.default_rng(abs(hash(shape)) % 2 ** 32 + (dtype == "c128"))
+    a, b, axes = contiguous_case(rng, shape, dtype=dtype, nq=1 + (shape[4] % 2))
+    ref = O.OraclePlan(a, b, *axes)
+    c = ref.output_tensor(dtype)
+    c.data = ref.execute()
+    exp = dense_reference(a, b, axes)
+    got = c.to_dense() if c.rank else np.asarray(c.data[0])   # a scalar output is one element
+    assert got.shape == exp.shape
+    np.testing.assert_allclose(got, exp, rtol=0, atol=1e-13 * max(1.0, np.abs(exp).max()))
