@@ -278,6 +278,16 @@ qlt_status qlt_conjugate(int64_t n, double *z, void *stream);
  * accumulate path, contract_contiguous_axes.h ScaleUntouchedOutputBlocks_). */
 qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,
                               const int64_t *lengths, const double *s, void *x, void *stream);
+/* qlt_scale_segments without any device allocation; it has no counterpart in
+ * the reference.  For callers that cannot allocate device memory (tight
+ * sweep loops, harnesses that forbid hipMalloc): the segment list is staged in
+ * a fixed 1 MiB device array of the library, one per GPU, held until the
+ * process ends, instead of a per-call allocation.  At most 65536 segments
+ * (QLT_ERR_INVALID_ARG beyond).  Same arguments and result as
+ * qlt_scale_segments; returns once the scaling is complete.  Calls that
+ * overlap on one GPU (from several host threads) share the array and race. */
+qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64_t *offsets,
+                                      const int64_t *lengths, const double *s, void *x, void *stream);
 
 /* Batched 2-D copy: rectangle i (rows[i] x cols[i] elements of dtype) from
  * src + src_offsets[i] with row stride src_ld[i] to dst + dst_offsets[i] with

@@ -147,6 +147,8 @@ _SIGS = {
     "qlt_real_to_complex": ([C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_conjugate": ([C.c_int64, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_scale_segments": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_scale_segments_noalloc": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p],
+                                   C.c_int32),
     "qlt_copy_blocks_2d": ([C.c_int32, C.c_int64, _i64p, _i64p, C.c_void_p, _i64p, _i64p, C.c_void_p, _i64p, _i64p,
                             C.c_void_p], C.c_int32),
     "qlt_block_svd": ([C.c_int32, C.c_int64, _i64p, _i64p, C.c_void_p, _i64p, C.c_void_p, _i64p, C.c_void_p, _i64p,

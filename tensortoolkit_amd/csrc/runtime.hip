@@ -13,9 +13,18 @@
 
 #include "qlt_internal.h"
 
+// Segment list of qlt_scale_segments_noalloc (offsets, then lengths), for
+// callers that cannot allocate device memory.  One copy per GPU, 1 MiB, held
+// until the process ends; its device address is looked up once per device in
+// qlt_init (the name itself is a host-side stand-in, never passed to a kernel).
+constexpr int64_t kNoAllocMaxSegs = 65536;
+static __device__ __align__(256) int64_t scratch[2 * kNoAllocMaxSegs];
+
 namespace qlt {
 
 namespace {
+constexpr int kMaxDevices = 64;
+int64_t *g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch`, per device
 thread_local std::string g_last_error;
 bool g_initialized = false;
 int g_device = 0;
@@ -324,6 +333,13 @@ qlt_status qlt_init(int32_t device) {
     return fail(QLT_ERR_INVALID_ARG, "qlt_init: device " + std::to_string(device) +
                                          " out of range (" + std::to_string(count) + " devices)");
   QLT_HIP(hipSetDevice(device));
+  if (device >= kMaxDevices)
+    return fail(QLT_ERR_UNSUPPORTED, "qlt_init: device " + std::to_string(device) + " beyond the supported 64");
+  if (g_scratch_ptr[device] == nullptr) {
+    void *p = nullptr;
+    QLT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<int64_t *>(p);
+  }
   if (g_initialized && g_device == device) return QLT_OK;
   if (g_reduce_buf != nullptr) {
     (void)hipFree(g_reduce_buf);
@@ -629,6 +645,40 @@ qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offs
   QLT_HIP(hipGetLastError());
   QLT_HIP(hipStreamSynchronize(st));
   QLT_HIP(hipFreeAsync(d, st));
+  QLT_HIP(hipStreamSynchronize(st));
+  return QLT_OK;
+}
+
+qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64_t *offsets,
+                                      const int64_t *lengths, const double *s, void *x, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_scale_segments_noalloc: bad dtype");
+  QLT_CHECK_ARG(s != nullptr, "qlt_scale_segments_noalloc: null scalar");
+  if (n_segs <= 0) return QLT_OK;
+  QLT_CHECK_ARG(n_segs <= kNoAllocMaxSegs,
+                "qlt_scale_segments_noalloc: " + std::to_string(n_segs) + " segments exceed the maximum of " +
+                    std::to_string(kNoAllocMaxSegs));
+  QLT_CHECK_ARG(offsets != nullptr && lengths != nullptr && x != nullptr,
+                "qlt_scale_segments_noalloc: null array");
+  hipStream_t st = as_stream(stream);
+  QLT_CHECK_ARG(!stream_capturing(st), "qlt_scale_segments_noalloc: not capturable (stages host arrays)");
+  int dev = -1;
+  QLT_HIP(hipGetDevice(&dev));
+  QLT_CHECK_ARG(dev >= 0 && dev < kMaxDevices && g_scratch_ptr[dev] != nullptr,
+                "qlt_scale_segments_noalloc: qlt_init has not run on the current device");
+  // the device's copy of the segment list; every entry the kernel reads is
+  // written here first, so nothing left by an earlier call is ever read
+  int64_t *d = g_scratch_ptr[dev];
+  QLT_HIP(hipMemcpyAsync(d, offsets, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  QLT_HIP(hipMemcpyAsync(d + kNoAllocMaxSegs, lengths, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL(k_scale_segments<false>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
+                       d, d + kNoAllocMaxSegs, s[0], 0.0, static_cast<double *>(x));
+  else
+    hipLaunchKernelGGL(k_scale_segments<true>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
+                       d, d + kNoAllocMaxSegs, s[0], s[1], static_cast<double *>(x));
+  QLT_HIP(hipGetLastError());
+  // the segment list is reused by the next call: return only once this one is done
   QLT_HIP(hipStreamSynchronize(st));
   return QLT_OK;
 }
