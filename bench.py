@@ -183,6 +183,35 @@ def time_graph(runner: MatvecRunner, hip: Hip, mode: int, replays: int):
     return ms / replays
 
 
+def config2_graph_replay(hip: Hip, stream, replays: int, tune: bool) -> dict:
+    """The lognormal-degeneracy U(1) FP64 matvec (sector degeneracies
+    lognormal(4, 1) clipped to 1..512): the 4-contraction chain captured once
+    as a hipGraph and replayed `replays` times, after the autotuner has chosen
+    tile shapes and k-slicing on the chain itself (untimed); the eager chain
+    under the same tuning beside it."""
+    mv2 = W.config2()
+    r2 = MatvecRunner(mv2, stream)
+    flops = float(sum(r2.flops))
+    out = {"workload": mv2.name, "dtype": "f64", "mode": "native", "replays": replays,
+           "gflop_per_step": round(flops / 1e9, 4), "tasks": [int(p.info.n_tasks) for p in r2.plans]}
+    TUNE.set_tuning(None)
+    if tune:
+        t0 = time.perf_counter()
+        res = TUNE.autotune(r2.tune_items(), r2.dtype, capi.QLT_MODE_NATIVE, reps=3, max_buckets=16, stream=stream)
+        out["autotune"] = {"trials": res.trials, "ms_before": round(res.ms_before, 4),
+                           "ms_after": round(res.ms_after, 4), "overrides": TUNE.overrides(res.tuning),
+                           "seconds": round(time.perf_counter() - t0, 1)}
+        r2.rebuild_plans()
+    total_ms, _, _, _ = time_mode(r2, hip, capi.QLT_MODE_NATIVE, 20, 3, int(np.argmax(r2.flops)))
+    graph_ms = time_graph(r2, hip, capi.QLT_MODE_NATIVE, replays)
+    out.update(ms_per_step_eager=round(total_ms / 20, 4), gflops_eager=round(flops / (total_ms / 20 * 1e-3) / 1e9, 2),
+               ms_per_replay=round(graph_ms, 4), gflops_replay=round(flops / (graph_ms * 1e-3) / 1e9, 2))
+    TUNE.set_tuning(None)
+    for p in r2.plans:
+        p.close()
+    return out
+
+
 # Slice products per algorithmic product in the split-BF16 GEMM: five BF16
 # slices per operand, every slice pair (i, j) with i + j <= 4.
 SPLIT_PRODUCTS = 15
@@ -215,10 +244,12 @@ def profile_traffic(kernel_name: str, workload: str):
 
 def kernel_roofline(kernel: dict, workload: str) -> dict:
     """Roofline of the probed kernel: algorithmic flops (sum of c*m*k*n over
-    its tasks) over its event-timed duration.  The split GEMM runs on BF16
-    MFMA, so its rate is given in BF16 flops (15 slice products per
-    algorithmic product) against the BF16 peak, with the FP64-equivalent
-    rate beside it."""
+    its tasks, the reference's AddGemmCost) over its event-timed duration.
+    A split GEMM runs on BF16 (int8) MFMA, so `achieved` (algorithmic flops
+    per second, no slice products counted) is set against the BF16 (int8)
+    peak; the rate of the slice products the units actually execute (15 BF16
+    or 21 int8 products per algorithmic product) is reported beside it, as
+    is the FP64-equivalent fraction of the FP64 peak."""
     secs = kernel["ms"] * 1e-3
     fp64_tf = kernel["flops"] / secs / 1e12
     split = kernel["name"].startswith("k_split_gemm")
@@ -233,13 +264,19 @@ def kernel_roofline(kernel: dict, workload: str) -> dict:
                    frac=round(gbs / PEAK_HBM_GBS, 4))
     elif split_i8:
         i8_tops = I8_PRODUCTS * fp64_tf
-        out.update(bound="mfma", achieved=round(i8_tops, 2), peak=PEAK_INT8_TOPS, unit="TOP/s",
-                   frac=round(i8_tops / PEAK_INT8_TOPS, 4), fp64_equivalent_tflops=round(fp64_tf, 3),
+        out.update(bound="mfma", achieved=round(fp64_tf, 3), peak=PEAK_INT8_TOPS, unit="TOP/s",
+                   frac=round(fp64_tf / PEAK_INT8_TOPS, 5),
+                   slice_products_per_algorithmic_product=I8_PRODUCTS,
+                   slice_product_rate=round(i8_tops, 2), slice_product_frac=round(i8_tops / PEAK_INT8_TOPS, 4),
+                   fp64_equivalent_tflops=round(fp64_tf, 3),
                    fp64_equivalent_frac_of_fp64_peak=round(fp64_tf / PEAK_FP64_TFLOPS, 4))
     elif split:
         bf16_tf = SPLIT_PRODUCTS * fp64_tf
-        out.update(bound="mfma", achieved=round(bf16_tf, 2), peak=PEAK_BF16_TFLOPS, unit="TFLOP/s",
-                   frac=round(bf16_tf / PEAK_BF16_TFLOPS, 4), fp64_equivalent_tflops=round(fp64_tf, 3),
+        out.update(bound="mfma", achieved=round(fp64_tf, 3), peak=PEAK_BF16_TFLOPS, unit="TFLOP/s",
+                   frac=round(fp64_tf / PEAK_BF16_TFLOPS, 5),
+                   slice_products_per_algorithmic_product=SPLIT_PRODUCTS,
+                   slice_product_rate=round(bf16_tf, 2), slice_product_frac=round(bf16_tf / PEAK_BF16_TFLOPS, 4),
+                   fp64_equivalent_tflops=round(fp64_tf, 3),
                    fp64_equivalent_frac_of_fp64_peak=round(fp64_tf / PEAK_FP64_TFLOPS, 4))
     else:
         out.update(bound="mfma", achieved=round(fp64_tf, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
@@ -295,8 +332,25 @@ def cpu_baseline(mv: W.Matvec, budget_s: float = 10.0):
         "cores": cores,
         "kind": "port",
         "sample": f"{done} full matvec step(s), execute only ({t_exec:.1f} s); plan build {plan_s:.2f} s "
-                  f"once; OpenBLAS zgemm per task, {info['threads']} threads ({info['library']})",
+                  f"once; OpenBLAS {'zgemm' if mv.dtype == 'c128' else 'dgemm'} per task, "
+                  f"{info['threads']} threads ({info['library']})",
+        "cpu_model": cpu_model(),
+        "blas": info.get("config") or info["library"],
+        "threads_note": f"{info['threads']} BLAS threads = the host cores this process may use; the reference's "
+                        "default CPU thread count is 4 (include/qlten/framework/consts.h:24)",
     }, result
+
+
+def cpu_model() -> str:
+    """Model name of the host CPU (first entry of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_e2e(args) -> int:
@@ -371,6 +425,9 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="run the autotuner on the chain per mode first (untimed) and time under its result; "
                          "default: the built-in tile rules")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the lognormal-degeneracy FP64 chain (autotuned, hipGraph replayed) reported beside "
+                         "the default workload")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (scales rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -441,6 +498,13 @@ def main() -> int:
         print(f"[bench] {name}: {total_ms / args.steps:.3f} ms/step, per launch {np.round(per_launch, 3).tolist()}, "
               f"dominant kernel {kernel['name']} {kernel['ms']:.4f} ms"
               + (f", graph replay {graph_ms:.3f} ms/step" if graph_ms else ""), file=sys.stderr, flush=True)
+
+    config2 = None
+    if args.config == "u1u1_D4096" and not args.no_config2 and args.graph_replays:
+        config2 = config2_graph_replay(hip, stream, args.graph_replays, tune=True)
+        print(f"[bench] {config2['workload']}: graph replay {config2['ms_per_replay']:.3f} ms "
+              f"({config2['gflops_replay']:.0f} GFLOP/s), eager {config2['ms_per_step_eager']:.3f} ms",
+              file=sys.stderr, flush=True)
 
     # correctness of the step output against the FP64 CPU oracle
     errors = {}
@@ -519,6 +583,7 @@ def main() -> int:
         "plan_build_s": round(runner.plan_seconds, 3),
         "autotune": {k: v["autotune"] for k, v in results.items()},
         "split_i8": split_i8_summary(),
+        "config2_graph_replay": config2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1733,8 +1733,7 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const
     if (post) hipLaunchKernelGGL(k_scale_tiles, dim3(n_mfma), blk, 0, st, rtiles, ar, ai, cd);
   }
   if (probe_split) {
-    double elems = 0.0;
-    for (int32_t cls = 0; cls < kNumMfmaClasses; ++cls) elems += plan->cls_elems[cls];
+    const double elems = plan->mfma_elems;   // distinct A and B blocks of all classes, plus C
     const char *name = i8 ? (cplx ? "k_split_i8_c2 c128" : "k_split_i8_f64 f64")
                           : (cplx ? "k_split_gemm_c2 c128" : "k_split_gemm_f64 f64");
     probe_end(st, name, (cplx ? 8.0 : 2.0) * mfma_mkn,

@@ -971,6 +971,10 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
       plan->cls_mkn[s.cls] += static_cast<double>(s.m) * static_cast<double>(s.n) * static_cast<double>(s.k_total);
       plan->cls_elems[s.cls] += static_cast<double>(s.m) * static_cast<double>(s.n);
       const uint8_t bit = static_cast<uint8_t>(1u << s.cls);
+      // bit 7: read by any MFMA class (the split GEMM runs all of them in one launch)
+      constexpr uint8_t kAnyMfma = 0x80;
+      const bool mfma = is_mfma_class(s.cls);
+      if (mfma) plan->mfma_elems += static_cast<double>(s.m) * static_cast<double>(s.n);
       for (int32_t t = s.t0; t < s.t1; ++t) {
         const ExecTask &e = plan->h_exec[static_cast<size_t>(t)];
         if (!(seen_a[e.a_blk] & bit)) {
@@ -980,6 +984,14 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
         if (!(seen_b[e.b_blk] & bit)) {
           seen_b[e.b_blk] |= bit;
           plan->cls_elems[s.cls] += static_cast<double>(e.k) * static_cast<double>(s.n);
+        }
+        if (mfma && !(seen_a[e.a_blk] & kAnyMfma)) {
+          seen_a[e.a_blk] |= kAnyMfma;
+          plan->mfma_elems += static_cast<double>(s.m) * static_cast<double>(e.k);
+        }
+        if (mfma && !(seen_b[e.b_blk] & kAnyMfma)) {
+          seen_b[e.b_blk] |= kAnyMfma;
+          plan->mfma_elems += static_cast<double>(e.k) * static_cast<double>(s.n);
         }
       }
     }

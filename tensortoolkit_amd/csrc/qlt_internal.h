@@ -290,6 +290,7 @@ struct qlt_plan {
   int64_t cls_tiles[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};  // tiles per class, class-major in d_tiles
   double cls_mkn[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};     // sum of m*k*n over the class's tasks
   double cls_elems[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};   // distinct A + B block elements + C elements
+  double mfma_elems = 0.0;              // the same over all MFMA classes together (blocks counted once)
   // ---- k-slicing of long MFMA tiles (native path) ----
   qlt::ReduceItem *d_red = nullptr;     // class-major reduction items
   int64_t cls_red[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};

@@ -119,10 +119,17 @@ HUBB_MPO_DEGS = [2, 1, 1, 1, 1]
 
 
 def config1(seed: int = 20261202, D: int = 2048, dtype: str = "f64") -> Matvec:
-    """D=2048 U(1) two-site matvec: charges -8..8, deg(q) ~ exp(-q^2/18)."""
+    """D=2048 U(1) two-site matvec: charges -8..8, deg(q) ~ exp(-q^2/18);
+    L and R symmetrised to be Hermitian (L[a',w,a] = conj L[a,w~,a'],
+    likewise R), as environments of a Hermitian Hamiltonian are."""
     charges, degs = u1_gaussian_degs(8, D)
-    return build_matvec([(int(q),) for q in charges], degs, [(0,), (1,)], HEIS_MPO_QNS, HEIS_MPO_DEGS,
-                        dtype, seed, f"u1_matvec_D{D}")
+    mv = build_matvec([(int(q),) for q in charges], degs, [(0,), (1,)], HEIS_MPO_QNS, HEIS_MPO_DEGS,
+                      dtype, seed, f"u1_matvec_D{D}")
+    wbar_l = _partner_sector(mv.L.indexes[1])
+    wbar_r = _partner_sector(mv.R.indexes[1])
+    mv.L = _symmetrize(mv.L, lambda c: (c[2], int(wbar_l[c[1]]), c[0]), (2, 1, 0))
+    mv.R = _symmetrize(mv.R, lambda c: (c[2], int(wbar_r[c[1]]), c[0]), (2, 1, 0))
+    return mv
 
 
 def config2(seed: int = 20261202, dtype: str = "f64") -> Matvec:

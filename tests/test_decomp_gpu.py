@@ -64,6 +64,21 @@ def test_svd_batched_shapes(qlt, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_svd_large_blocks(qlt, dtype):
+    """Block matrices of the end-to-end configuration's sizes, up to 1024 x
+    1024 (square, tall, wide), in one batched call."""
+    rng = np.random.default_rng(1024 + (dtype == "c128"))
+    shapes = [(512, 512), (1024, 1024), (1024, 600), (600, 1024), (777, 513)]
+    mats = [rand_mat(rng, m, n, dtype == "c128") for m, n in shapes]
+    _say(f"case: svd large blocks {dtype} shapes {shapes}")
+    out, sweeps = DEC.block_svd_dense(mats, dtype)
+    _say(f"sweeps {sweeps}")
+    assert sweeps <= 40
+    for a, (u, s, vt) in zip(mats, out):
+        check_svd(a, u, s, vt)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_svd_graded_and_rank_deficient(qlt, dtype):
     rng = np.random.default_rng(7)
     cp = dtype == "c128"

@@ -44,3 +44,21 @@ def test_degeneracy_split_needs_one_per_sector():
     with pytest.raises(ValueError):
         W.largest_remainder([1.0, 1.0, 1.0], 2)
     assert W.largest_remainder([1.0, 2.0, 1.0], 8).sum() == 8
+
+
+def test_config1_environments_are_hermitian():
+    """The D = 2048 matvec's L and R are symmetrised: as dense arrays,
+    L[a', w, a] = conj L[a, w~, a'] with w~ the conjugate MPO sector (the
+    Heisenberg bond sectors -1 and +1 swap, 0 stays), and likewise R."""
+    mv = W.config1()
+    for t in (mv.L, mv.R):
+        d = t.to_dense()
+        ix = t.indexes[1]
+        offs = ix.sector_offsets()
+        perm = np.empty(ix.dim, dtype=np.int64)
+        part = W._partner_sector(ix)
+        for k in range(ix.nsec):
+            for j in range(ix.degs[k]):
+                perm[offs[k] + j] = offs[part[k]] + j
+        np.testing.assert_array_equal(d.transpose(2, 1, 0)[:, perm, :], np.conj(d))
+        assert not np.array_equal(d, d.transpose(2, 1, 0))   # not trivially symmetric
