@@ -289,6 +289,39 @@ qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offs
 qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64_t *offsets,
                                       const int64_t *lengths, const double *s, void *x, void *stream);
 
+/* ---- device-resident Krylov primitives over a basis arena ----
+ * The vector operations of the reference's Lanczos solver on tensors --
+ * InnerProduct (tensor_manipulation/basic_operations.h:146-203), LinearCombine
+ * (ten_linear_combine.h:186), Normalize (qltensor/qltensor_impl.h:568-573) --
+ * on raw data, with every scalar and the number of active basis vectors in
+ * device memory, so that a whole Lanczos iteration (matvec chain, two passes
+ * of classical Gram-Schmidt against the basis, normalisation) launches with
+ * no host round trip and can be captured once as a graph and replayed.
+ * The arena holds k_max vectors of n elements, vector i at V + i * ldv
+ * (elements of dtype; ldv >= n); `count` points to an int32 in device memory,
+ * the number m of active vectors (clamped to 0 .. k_max), 1 <= k_max <= 1024.
+ * Reductions are two-pass with a fixed grid and summation order: results are
+ * bitwise reproducible.  None of these calls synchronises.
+ *
+ * qlt_basis_dot:    c[2i], c[2i+1] = <V_i, w> = sum conj(V_i) w for i < m,
+ *                   0 for m <= i < k_max (c: 2 k_max doubles, device);
+ *                   work: QLT_BASIS_WORK_DOUBLES(k_max) doubles, device.
+ * qlt_basis_update: w += scale * sum_{i<m} c_i V_i (c complex pairs, device).
+ * qlt_lanczos_extend: with j = m - 1: alphas[j] = Re c[2j], betas[j] = |w|,
+ *                   V_m = w / |w| (zero when |w| = 0) if m < k_max, then
+ *                   m = m + 1 (when m < k_max).  work as for qlt_basis_dot.
+ * qlt_basis_select: x = V_{*index + offset} (no-op outside 0 .. k_max-1). */
+#define QLT_BASIS_WORK_DOUBLES(k_max) (2 * 512 * (k_max) + 1)
+qlt_status qlt_basis_dot(int32_t dtype, int64_t n, int32_t k_max, const int32_t *count, const void *V, int64_t ldv,
+                         const void *w, double *c, double *work, void *stream);
+qlt_status qlt_basis_update(int32_t dtype, int64_t n, int32_t k_max, const int32_t *count, double scale,
+                            const double *c, const void *V, int64_t ldv, void *w, void *stream);
+qlt_status qlt_lanczos_extend(int32_t dtype, int64_t n, int32_t k_max, int32_t *count, const double *c,
+                              const void *w, void *V, int64_t ldv, double *alphas, double *betas, double *work,
+                              void *stream);
+qlt_status qlt_basis_select(int32_t dtype, int64_t n, int32_t k_max, const int32_t *index, int32_t offset,
+                            const void *V, int64_t ldv, void *x, void *stream);
+
 /* Batched 2-D copy: rectangle i (rows[i] x cols[i] elements of dtype) from
  * src + src_offsets[i] with row stride src_ld[i] to dst + dst_offsets[i] with
  * row stride dst_ld[i]; host arrays, counted in elements.  Assembles block

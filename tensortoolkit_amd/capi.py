@@ -149,6 +149,14 @@ _SIGS = {
     "qlt_scale_segments": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_scale_segments_noalloc": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p],
                                    C.c_int32),
+    "qlt_basis_dot": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_basis_update": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                          C.c_int64, C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_lanczos_extend": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_basis_select": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64,
+                          C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_copy_blocks_2d": ([C.c_int32, C.c_int64, _i64p, _i64p, C.c_void_p, _i64p, _i64p, C.c_void_p, _i64p, _i64p,
                             C.c_void_p], C.c_int32),
     "qlt_block_svd": ([C.c_int32, C.c_int64, _i64p, _i64p, C.c_void_p, _i64p, C.c_void_p, _i64p, C.c_void_p, _i64p,

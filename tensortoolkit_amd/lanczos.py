@@ -133,3 +133,116 @@ def lanczos(hv: DeviceMatvec, psi0: np.ndarray, steps: int = 20) -> LanczosResul
     vec.axpby(1.0 / nrm, gs.ptr, 0.0, gs.ptr)
     capi.check(lib.qlt_stream_sync(hv.stream))
     return LanczosResult(float(evals[0]), np.array(alphas), np.array(betas), gs, time.perf_counter() - t0, tmv)
+
+
+class KrylovArena:
+    """Device buffers of a Lanczos run whose iteration launches with no host
+    round trip: the basis arena (steps + 1 vectors), the active-vector count,
+    the Gram-Schmidt coefficients of both passes, the Lanczos coefficients and
+    the reduction workspace, all in device memory."""
+
+    def __init__(self, hv: DeviceMatvec, steps: int):
+        if hv.stream is None:
+            raise ValueError("KrylovArena needs a matvec on a created stream (graph capture cannot use the null stream)")
+        self.hv, self.steps = hv, steps
+        self.k_max = steps + 1
+        self.n, esz = hv.n, hv.esz
+        self.dt = capi.QLT_C128 if hv.dtype == "c128" else capi.QLT_F64
+        self.V = capi.DeviceBuffer(self.k_max * self.n * esz)
+        self.x = capi.DeviceBuffer(self.n * esz)
+        self.w = capi.DeviceBuffer(self.n * esz)
+        self.count = capi.DeviceBuffer(4)
+        self.c1 = capi.DeviceBuffer(16 * self.k_max)
+        self.c2 = capi.DeviceBuffer(16 * self.k_max)
+        self.alphas = capi.DeviceBuffer(8 * self.k_max)
+        self.betas = capi.DeviceBuffer(8 * self.k_max)
+        self.work = capi.DeviceBuffer(8 * (2 * 512 * self.k_max + 1))
+
+    def reset(self, psi0: np.ndarray) -> None:
+        """V_0 = psi0 / |psi0|, one active vector, coefficients cleared."""
+        lib, st = capi.lib(), self.hv.stream
+        v0 = np.ascontiguousarray(psi0 / np.sqrt(np.vdot(psi0, psi0).real))
+        for buf in (self.V, self.alphas, self.betas, self.c1, self.c2):
+            capi.check(lib.qlt_memset_zero(buf.ptr, buf.nbytes, st))
+        capi.check(lib.qlt_stream_sync(st))
+        capi.check(lib.qlt_memcpy_h2d(self.V.ptr, v0.ctypes.data_as(C.c_void_p), v0.nbytes))
+        one = np.array([1], dtype=np.int32)
+        capi.check(lib.qlt_memcpy_h2d(self.count.ptr, one.ctypes.data_as(C.c_void_p), 4))
+
+    def iteration(self) -> None:
+        """One Lanczos iteration, all on the matvec's stream: x = V_{m-1};
+        w = H x; two passes of w -= V (V^H w) over the m active vectors (the
+        first pass's <V_{m-1}, w> is alpha); beta = |w|; V_m = w / beta."""
+        lib, st, n, k, dt = capi.lib(), self.hv.stream, self.n, self.k_max, self.dt
+        capi.check(lib.qlt_basis_select(dt, n, k, self.count.ptr, -1, self.V.ptr, n, self.x.ptr, st))
+        self.hv.apply(self.x.ptr, self.w.ptr)
+        for c in (self.c1, self.c2):
+            capi.check(lib.qlt_basis_dot(dt, n, k, self.count.ptr, self.V.ptr, n, self.w.ptr, c.ptr,
+                                         self.work.ptr, st))
+            capi.check(lib.qlt_basis_update(dt, n, k, self.count.ptr, -1.0, c.ptr, self.V.ptr, n, self.w.ptr, st))
+        capi.check(lib.qlt_lanczos_extend(dt, n, k, self.count.ptr, self.c1.ptr, self.w.ptr, self.V.ptr, n,
+                                          self.alphas.ptr, self.betas.ptr, self.work.ptr, st))
+
+    def coefficients(self):
+        capi.check(capi.lib().qlt_stream_sync(self.hv.stream))
+        return (self.alphas.to_numpy(np.float64)[:self.steps], self.betas.to_numpy(np.float64)[:self.steps])
+
+    def ritz_vector(self, y: np.ndarray) -> capi.DeviceBuffer:
+        """sum_i y_i V_i, normalised, on the device."""
+        lib, st, n, dt = capi.lib(), self.hv.stream, self.n, self.dt
+        k = len(y)
+        gs = capi.DeviceBuffer(n * self.hv.esz)
+        capi.check(lib.qlt_memset_zero(gs.ptr, gs.nbytes, st))
+        coef = np.zeros(2 * k, dtype=np.float64)
+        coef[0::2] = y
+        cbuf = capi.DeviceBuffer.from_numpy(coef)
+        cnt = capi.DeviceBuffer.from_numpy(np.array([k], dtype=np.int32))
+        capi.check(lib.qlt_basis_update(dt, n, k, cnt.ptr, 1.0, cbuf.ptr, self.V.ptr, n, gs.ptr, st))
+        vec = DeviceVectors(self.hv.dtype, n, st)
+        nrm = np.sqrt(vec.dot(gs.ptr, gs.ptr).real)
+        vec.axpby(1.0 / nrm, gs.ptr, 0.0, gs.ptr)
+        capi.check(lib.qlt_stream_sync(st))
+        return gs
+
+
+def lanczos_graph(hv: DeviceMatvec, psi0: np.ndarray, steps: int = 20, use_graph: bool = True,
+                  arena: "KrylovArena | None" = None) -> LanczosResult:
+    """`steps` Lanczos iterations whose every vector operation stays on the
+    device (KrylovArena).  With `use_graph` one iteration is captured once as
+    a hipGraph and replayed `steps` times; otherwise the same launches run
+    eagerly.  Classical Gram-Schmidt against the whole basis, twice, takes
+    the place of the three-term recurrence plus reorthogonalisation (the same
+    projections in exact arithmetic).  The tridiagonal matrix is read back
+    once at the end and solved in FP64 on the host."""
+    lib = capi.lib()
+    ar = arena if arena is not None else KrylovArena(hv, steps)
+    ar.reset(psi0)
+    t0 = time.perf_counter()
+    if use_graph:
+        capi.check(lib.qlt_graph_begin(hv.stream))
+        try:
+            ar.iteration()
+        finally:
+            g = C.c_void_p()
+            capi.check(lib.qlt_graph_end(hv.stream, C.byref(g)))
+        try:
+            for _ in range(steps):
+                capi.check(lib.qlt_graph_replay(g, hv.stream))
+            capi.check(lib.qlt_stream_sync(hv.stream))
+        finally:
+            capi.check(lib.qlt_graph_destroy(g))
+    else:
+        for _ in range(steps):
+            ar.iteration()
+        capi.check(lib.qlt_stream_sync(hv.stream))
+    iter_s = time.perf_counter() - t0
+    alphas, betas = ar.coefficients()
+    k = steps
+    for j in range(steps - 1):        # an invariant subspace ends the recurrence
+        if betas[j] == 0.0:
+            k = j + 1
+            break
+    T = np.diag(alphas[:k]) + np.diag(betas[:k - 1], 1) + np.diag(betas[:k - 1], -1)
+    evals, evecs = np.linalg.eigh(T)
+    gs = ar.ritz_vector(evecs[:, 0])
+    return LanczosResult(float(evals[0]), alphas[:k], betas[:k - 1], gs, time.perf_counter() - t0, iter_s)
