@@ -365,8 +365,12 @@ class QLTensor {
     AllowedBlocks(div);
     const int64_t n = static_cast<int64_t>(bsdt_->RawDataSize()) * (IsComplex<ElemT>() ? 2 : 1);
     auto &st = GlobalRandomState();
-    QltCheck(qlt_fill_uniform(bsdt_->GetActualRawDataPtr(), n, st.seed, st.counter, 0.0, 1.0,
-                              hp_numeric::LibStream()));
+    if constexpr (IsSingle<ElemT>())
+      QltCheck(qlt_fill_uniform_f32(bsdt_->GetActualRawDataPtr(), n, st.seed, st.counter, 0.0, 1.0,
+                                    hp_numeric::LibStream()));
+    else
+      QltCheck(qlt_fill_uniform(bsdt_->GetActualRawDataPtr(), n, st.seed, st.counter, 0.0, 1.0,
+                                hp_numeric::LibStream()));
     st.counter += n;
   }
 
@@ -394,18 +398,19 @@ class QLTensor {
     return CalcDiv(indexes_, bsdt_->GetBlkIdxDataBlkMap().begin()->second.blk_coors);
   }
 
-  // sqrt(sum |x|^2) over the raw data.
-  double Get2Norm() const {
-    if (IsDefault() || bsdt_->GetActualRawDataSize() == 0) return 0.0;
+  using RealT = typename RealTypeTrait<ElemT>::type;
+  // sqrt(sum |x|^2) over the raw data (accumulated in FP64).
+  RealT Get2Norm() const {
+    if (IsDefault() || bsdt_->GetActualRawDataSize() == 0) return RealT(0);
     double out[2];
     const void *p = bsdt_->GetActualRawDataPtr();
     QltCheck(qlt_dot(DType(), static_cast<int64_t>(bsdt_->GetActualRawDataSize()), p, p, out,
                      hp_numeric::LibStream()));
-    return std::sqrt(out[0]);
+    return static_cast<RealT>(std::sqrt(out[0]));
   }
-  double Normalize() {
-    const double norm = Get2Norm();
-    if (norm > 0.0) *this *= ElemT(1.0 / norm);
+  RealT Normalize() {
+    const RealT norm = Get2Norm();
+    if (norm > RealT(0)) *this *= ElemT(RealT(1) / norm);
     return norm;
   }
 
@@ -430,7 +435,9 @@ class QLTensor {
   QLTensor operator-(const QLTensor &rhs) const { return AxpbyUnion(ElemT(1.0), *this, ElemT(-1.0), rhs); }
   QLTensor &operator+=(const QLTensor &rhs) { return *this = *this + rhs; }
 
-  static constexpr int32_t DType() { return IsComplex<ElemT>() ? QLT_C128 : QLT_F64; }
+  static constexpr int32_t DType() {
+    return IsSingle<ElemT>() ? (IsComplex<ElemT>() ? QLT_C64 : QLT_F32) : (IsComplex<ElemT>() ? QLT_C128 : QLT_F64);
+  }
 
   // a * x + b * y over the union of the two block maps.
   static QLTensor AxpbyUnion(const ElemT a, const QLTensor &x, const ElemT b, const QLTensor &y) {

@@ -7,17 +7,21 @@
 #define QLTEN_AMD_TENSOR_MANIPULATION_BASIC_OPERATIONS_H
 
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "qlten/qltensor/qltensor.h"
 
 namespace qlten {
 
-template <typename QNT>
-QLTensor<QLTEN_Complex, QNT> ToComplex(const QLTensor<QLTEN_Double, QNT> &real_t) {
-  QLTensor<QLTEN_Complex, QNT> out(real_t.GetIndexes());
-  if (real_t.IsDefault()) return out;
-  const auto &src = real_t.GetBlkSparDataTen();
+namespace detail {
+// The same block layout with the data converted element by element
+// (real -> complex or a precision change).
+template <typename To, typename From, typename QNT>
+QLTensor<To, QNT> ConvertElems(const QLTensor<From, QNT> &t) {
+  QLTensor<To, QNT> out(t.GetIndexes());
+  if (t.IsDefault()) return out;
+  const auto &src = t.GetBlkSparDataTen();
   auto &dst = out.GetBlkSparDataTen();
   std::vector<CoorsT> coors;
   for (const auto &kv : src.GetBlkIdxDataBlkMap()) coors.push_back(kv.second.blk_coors);
@@ -25,9 +29,20 @@ QLTensor<QLTEN_Complex, QNT> ToComplex(const QLTensor<QLTEN_Double, QNT> &real_t
   else dst.DataBlksInsert(coors);
   if (src.GetActualRawDataSize() == 0) return out;
   dst.Allocate();
-  QltCheck(qlt_real_to_complex(static_cast<int64_t>(src.GetActualRawDataSize()), src.GetActualRawDataPtr(),
-                               reinterpret_cast<double *>(dst.GetActualRawDataPtr()), hp_numeric::LibStream()));
+  QltCheck(qlt_convert(QLTensor<From, QNT>::DType(), QLTensor<To, QNT>::DType(),
+                       static_cast<int64_t>(src.GetActualRawDataSize()), src.GetActualRawDataPtr(),
+                       dst.GetActualRawDataPtr(), hp_numeric::LibStream()));
   return out;
+}
+}  // namespace detail
+
+template <typename QNT>
+QLTensor<QLTEN_Complex, QNT> ToComplex(const QLTensor<QLTEN_Double, QNT> &real_t) {
+  return detail::ConvertElems<QLTEN_Complex>(real_t);
+}
+template <typename QNT>
+QLTensor<QLTEN_ComplexFloat, QNT> ToComplex(const QLTensor<QLTEN_Float, QNT> &real_t) {
+  return detail::ConvertElems<QLTEN_ComplexFloat>(real_t);
 }
 
 template <typename ElemT, typename QNT>
@@ -48,9 +63,12 @@ QLTensor<ElemT, QNT> Dag(const QLTensor<ElemT, QNT> &t) {
     hp_numeric::SyncLibStream();
     QltCheck(qlt_memcpy_d2d(dst.GetActualRawDataPtr(), src.GetActualRawDataPtr(),
                             src.GetActualRawDataSize() * sizeof(ElemT)));
-    if (IsComplex<ElemT>())
+    if constexpr (std::is_same<ElemT, QLTEN_Complex>::value)
       QltCheck(qlt_conjugate(static_cast<int64_t>(src.GetActualRawDataSize()),
                              reinterpret_cast<double *>(dst.GetActualRawDataPtr()), hp_numeric::LibStream()));
+    else if constexpr (std::is_same<ElemT, QLTEN_ComplexFloat>::value)
+      QltCheck(qlt_conjugate_c64(static_cast<int64_t>(src.GetActualRawDataSize()),
+                                 reinterpret_cast<float *>(dst.GetActualRawDataPtr()), hp_numeric::LibStream()));
   }
   return shell;
 }
@@ -80,8 +98,9 @@ ElemT InnerProduct(const QLTensor<ElemT, QNT> &a, const QLTensor<ElemT, QNT> &b)
   if (sa.GetActualRawDataSize())
     QltCheck(qlt_dot(QLTensor<ElemT, QNT>::DType(), static_cast<int64_t>(sa.GetActualRawDataSize()),
                      sa.GetActualRawDataPtr(), sb.GetActualRawDataPtr(), out, hp_numeric::LibStream()));
-  if constexpr (IsComplex<ElemT>()) return ElemT(out[0], out[1]);
-  else return out[0];
+  using R = typename RealTypeTrait<ElemT>::type;
+  if constexpr (IsComplex<ElemT>()) return ElemT(static_cast<R>(out[0]), static_cast<R>(out[1]));
+  else return static_cast<R>(out[0]);
 }
 
 }  // namespace qlten

@@ -227,6 +227,20 @@ void Contract(const QLTensor<QLTEN_Complex, QNT> *pa, const QLTensor<QLTEN_Doubl
   Contract(pa, &cb, axes_set, pc);
 }
 
+template <typename QNT>
+void Contract(const QLTensor<QLTEN_Float, QNT> *pa, const QLTensor<QLTEN_ComplexFloat, QNT> *pb,
+              const std::vector<std::vector<size_t>> &axes_set, QLTensor<QLTEN_ComplexFloat, QNT> *pc) {
+  auto ca = ToComplex(*pa);
+  Contract(&ca, pb, axes_set, pc);
+}
+
+template <typename QNT>
+void Contract(const QLTensor<QLTEN_ComplexFloat, QNT> *pa, const QLTensor<QLTEN_Float, QNT> *pb,
+              const std::vector<std::vector<size_t>> &axes_set, QLTensor<QLTEN_ComplexFloat, QNT> *pc) {
+  auto cb = ToComplex(*pb);
+  Contract(pa, &cb, axes_set, pc);
+}
+
 // Contract(A, axes_a, B, axes_b, C) -- ten_ctrct.h:352-362.
 template <typename TenElemT1, typename TenElemT2, typename TenElemT3, typename QNT>
 void Contract(const QLTensor<TenElemT1, QNT> *pa, const std::vector<size_t> &axes_a,

@@ -170,11 +170,12 @@ void SVD(const QLTensor<TenElemT, QNT> *pt, const size_t ldims, const QNT &lqndi
   // S: diagonal blocks holding each matrix's kept singular values
   auto &sb = ps->GetBlkSparDataTen();
   sb.DataBlksInsert(s_coors);
-  std::vector<double> sdata(sb.RawDataSize(), 0.0);
+  using SRealT = typename RealTypeTrait<TenElemT>::type;
+  std::vector<SRealT> sdata(sb.RawDataSize(), SRealT(0));
   for (size_t coor = 0; coor < live.size(); ++coor) {
     const size_t i = live[coor], k = kept[i];
     const size_t off = sb.GetBlkIdxDataBlkMap().at(sb.BlkCoorsToBlkIdx(s_coors[coor])).data_offset;
-    for (size_t j = 0; j < k; ++j) sdata[off + j * k + j] = svals[i][j];
+    for (size_t j = 0; j < k; ++j) sdata[off + j * k + j] = static_cast<SRealT>(svals[i][j]);
   }
   sb.RawDataFromHost(sdata);
   *pactual_trunc_err = static_cast<typename RealTypeTrait<TenElemT>::type>(err);

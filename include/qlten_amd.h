@@ -48,8 +48,12 @@ enum {
   QLT_ERR_NOT_CONVERGED = 8
 };
 
-/* Element types (reference QLTEN_Double / QLTEN_Complex, framework/value_t.h:37-81). */
-enum { QLT_F64 = 0, QLT_C128 = 1 };
+/* Element types (reference QLTEN_Double / QLTEN_Complex / QLTEN_Float /
+ * QLTEN_ComplexFloat, framework/value_t.h:37-81).  The single-precision types
+ * are stored as float (complex: interleaved float pairs) and contracted with
+ * FP64 arithmetic, rounded to float once per output element; the split modes
+ * (which emulate FP64) run them on the native kernels. */
+enum { QLT_F64 = 0, QLT_C128 = 1, QLT_F32 = 2, QLT_C64 = 3 };
 
 /* Arithmetic of the block GEMMs.
  *   QLT_MODE_NATIVE      FP64 MFMA (complex128 as four real FP64 MFMA products).
@@ -263,7 +267,8 @@ qlt_status qlt_graph_destroy(qlt_graph *graph);
  * (raw_data_operations.h:141-166), whose stream is implementation-defined. */
 qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t first_index,
                             double lo, double hi, void *stream);
-/* y = a*x + b*y over n elements (LinearCombine building block, ten_linear_combine.h:186). */
+/* y = a*x + b*y over n elements of any element type, FP64 arithmetic
+ * (LinearCombine building block, ten_linear_combine.h:186). */
 qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x,
                      const double *b, void *y, void *stream);
 /* out[0..1] = sum conj(x_i) * y_i (InnerProduct, basic_operations.h:203); real: out[1]=0.
@@ -272,8 +277,20 @@ qlt_status qlt_dot(int32_t dtype, int64_t n, const void *x, const void *y, doubl
                    void *stream);
 /* Real -> complex promotion (VectorRealToCplxKernel, framework/hp_numeric/blas_level1.h:825-843). */
 qlt_status qlt_real_to_complex(int64_t n, const double *x, double *z, void *stream);
-/* Complex conjugation in place (ConjugateKernel, utility/utils_inl.h:556-571). */
+/* Complex conjugation in place (ConjugateKernel, utility/utils_inl.h:556-571);
+ * n complex elements, complex128 or (qlt_conjugate_c64) complex64. */
 qlt_status qlt_conjugate(int64_t n, double *z, void *stream);
+qlt_status qlt_conjugate_c64(int64_t n, float *z, void *stream);
+/* The seeded fill of qlt_fill_uniform with each value rounded to float
+ * (Random of QLTEN_Float / QLTEN_ComplexFloat tensors). */
+qlt_status qlt_fill_uniform_f32(void *x, int64_t n_floats, uint64_t seed, int64_t first_index,
+                                double lo, double hi, void *stream);
+/* Element conversion of n elements between element types: precision changes
+ * (rounded to nearest) and real -> complex (imaginary part 0; ToComplex,
+ * tensor_manipulation/basic_operations.h:98-130); complex -> real is
+ * rejected. */
+qlt_status qlt_convert(int32_t src_dtype, int32_t dst_dtype, int64_t n, const void *src, void *dst,
+                       void *stream);
 /* Scale selected blocks: x[off_i .. off_i+len_i) *= s  (untouched-block scaling of the
  * accumulate path, contract_contiguous_axes.h ScaleUntouchedOutputBlocks_). */
 qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,

@@ -16,7 +16,7 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libqlten_amd.so"
 
 QLT_OK = 0
-QLT_F64, QLT_C128 = 0, 1
+QLT_F64, QLT_C128, QLT_F32, QLT_C64 = 0, 1, 2, 3
 QLT_MODE_NATIVE, QLT_MODE_SPLIT_BF16, QLT_MODE_SPLIT_I8 = 0, 1, 2
 QLT_MAX_RANK = 8
 
@@ -149,6 +149,10 @@ _SIGS = {
     "qlt_scale_segments": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_scale_segments_noalloc": ([C.c_int32, C.c_int64, _i64p, _i64p, _f64p, C.c_void_p, C.c_void_p],
                                    C.c_int32),
+    "qlt_conjugate_c64": ([C.c_int64, C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_fill_uniform_f32": ([C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p],
+                             C.c_int32),
+    "qlt_convert": ([C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_basis_dot": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                        C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_basis_update": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,

@@ -680,6 +680,22 @@ qlt_status qr_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, con
   return QLT_OK;
 }
 
+// Single-precision element types: the block matrices are widened to FP64
+// (complex128) device buffers, decomposed there, and the factors rounded
+// back; `ext` is the extent (elements) of a buffer addressed by offsets.
+int64_t extent(int64_t n, const int64_t *offs, const int64_t *r, const int64_t *c) {
+  int64_t e = 0;
+  for (int64_t i = 0; i < n; ++i) e = std::max(e, offs[i] + r[i] * c[i]);
+  return e;
+}
+
+struct DevTmp {
+  void *p = nullptr;
+  ~DevTmp() {
+    if (p != nullptr) (void)hipFree(p);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -689,7 +705,7 @@ qlt_status qlt_block_svd(int32_t dtype, int64_t n_mats, const int64_t *rows, con
                          const int64_t *s_offsets, void *vt, const int64_t *vt_offsets, int32_t *sweeps,
                          void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_block_svd: bad dtype");
+  QLT_CHECK_ARG(qlt::valid_dtype(dtype), "qlt_block_svd: bad dtype");
   QLT_TRY(check_shapes("qlt_block_svd", n_mats, rows, cols));
   if (sweeps != nullptr) *sweeps = 0;
   if (n_mats == 0) return QLT_OK;
@@ -697,6 +713,26 @@ qlt_status qlt_block_svd(int32_t dtype, int64_t n_mats, const int64_t *rows, con
   QLT_CHECK_ARG(a_offsets != nullptr && u_offsets != nullptr && s_offsets != nullptr && vt_offsets != nullptr,
                 "qlt_block_svd: null offset array");
   QLT_CHECK_ARG(!qlt::stream_capturing(qlt::as_stream(stream)), "qlt_block_svd: not capturable (host-driven sweeps)");
+  if (qlt::dtype_is_single(dtype)) {
+    const int32_t wide = dtype == QLT_F32 ? QLT_F64 : QLT_C128;
+    const int64_t esz = qlt::dtype_bytes(wide);
+    std::vector<int64_t> kk(static_cast<size_t>(n_mats));
+    for (int64_t i = 0; i < n_mats; ++i) kk[i] = std::min(rows[i], cols[i]);
+    const int64_t na = extent(n_mats, a_offsets, rows, cols);
+    const int64_t nu = extent(n_mats, u_offsets, rows, kk.data());
+    const int64_t nv = extent(n_mats, vt_offsets, kk.data(), cols);
+    DevTmp wa, wu, wv;
+    QLT_HIP(hipMalloc(&wa.p, std::max<int64_t>(1, na) * esz));
+    QLT_HIP(hipMalloc(&wu.p, std::max<int64_t>(1, nu) * esz));
+    QLT_HIP(hipMalloc(&wv.p, std::max<int64_t>(1, nv) * esz));
+    QLT_TRY(qlt_convert(dtype, wide, na, a, wa.p, stream));
+    QLT_TRY(qlt_block_svd(wide, n_mats, rows, cols, wa.p, a_offsets, wu.p, u_offsets, s, s_offsets, wv.p,
+                          vt_offsets, sweeps, stream));
+    QLT_TRY(qlt_convert(wide, dtype, nu, wu.p, u, stream));
+    QLT_TRY(qlt_convert(wide, dtype, nv, wv.p, vt, stream));
+    QLT_HIP(hipStreamSynchronize(qlt::as_stream(stream)));
+    return QLT_OK;
+  }
   if (dtype == QLT_F64)
     return svd_impl<false>(n_mats, rows, cols, a, a_offsets, u, u_offsets, s, s_offsets, vt, vt_offsets, sweeps,
                            qlt::as_stream(stream));
@@ -708,13 +744,32 @@ qlt_status qlt_block_qr(int32_t dtype, int64_t n_mats, const int64_t *rows, cons
                         const int64_t *a_offsets, void *q, const int64_t *q_offsets, void *r,
                         const int64_t *r_offsets, void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_block_qr: bad dtype");
+  QLT_CHECK_ARG(qlt::valid_dtype(dtype), "qlt_block_qr: bad dtype");
   QLT_TRY(check_shapes("qlt_block_qr", n_mats, rows, cols));
   if (n_mats == 0) return QLT_OK;
   QLT_CHECK_ARG(a != nullptr && q != nullptr && r != nullptr, "qlt_block_qr: null data pointer");
   QLT_CHECK_ARG(a_offsets != nullptr && q_offsets != nullptr && r_offsets != nullptr,
                 "qlt_block_qr: null offset array");
   QLT_CHECK_ARG(!qlt::stream_capturing(qlt::as_stream(stream)), "qlt_block_qr: not capturable (allocates)");
+  if (qlt::dtype_is_single(dtype)) {
+    const int32_t wide = dtype == QLT_F32 ? QLT_F64 : QLT_C128;
+    const int64_t esz = qlt::dtype_bytes(wide);
+    std::vector<int64_t> kk(static_cast<size_t>(n_mats));
+    for (int64_t i = 0; i < n_mats; ++i) kk[i] = std::min(rows[i], cols[i]);
+    const int64_t na = extent(n_mats, a_offsets, rows, cols);
+    const int64_t nq = extent(n_mats, q_offsets, rows, kk.data());
+    const int64_t nr = extent(n_mats, r_offsets, kk.data(), cols);
+    DevTmp wa, wq, wr;
+    QLT_HIP(hipMalloc(&wa.p, std::max<int64_t>(1, na) * esz));
+    QLT_HIP(hipMalloc(&wq.p, std::max<int64_t>(1, nq) * esz));
+    QLT_HIP(hipMalloc(&wr.p, std::max<int64_t>(1, nr) * esz));
+    QLT_TRY(qlt_convert(dtype, wide, na, a, wa.p, stream));
+    QLT_TRY(qlt_block_qr(wide, n_mats, rows, cols, wa.p, a_offsets, wq.p, q_offsets, wr.p, r_offsets, stream));
+    QLT_TRY(qlt_convert(wide, dtype, nq, wq.p, q, stream));
+    QLT_TRY(qlt_convert(wide, dtype, nr, wr.p, r, stream));
+    QLT_HIP(hipStreamSynchronize(qlt::as_stream(stream)));
+    return QLT_OK;
+  }
   if (dtype == QLT_F64)
     return qr_impl<false>(n_mats, rows, cols, a, a_offsets, q, q_offsets, r, r_offsets, qlt::as_stream(stream));
   return qr_impl<true>(n_mats, rows, cols, a, a_offsets, q, q_offsets, r, r_offsets, qlt::as_stream(stream));

@@ -22,9 +22,10 @@ extern "C" {
 qlt_status qlt_plan_prepare(qlt_plan *plan, int32_t dtype, int32_t mode) {
   QLT_REQUIRE_INIT();
   QLT_CHECK_ARG(plan != nullptr, "qlt_plan_prepare: null plan");
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_plan_prepare: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_plan_prepare: bad dtype");
   QLT_CHECK_ARG(valid_mode(mode), "qlt_plan_prepare: bad mode");
-  if (mode == QLT_MODE_NATIVE || plan->n_tiles == 0) return QLT_OK;
+  // single-precision element types always run natively (the split modes emulate FP64)
+  if (mode == QLT_MODE_NATIVE || plan->n_tiles == 0 || dtype_is_single(dtype)) return QLT_OK;
   if (!split_profitable(plan, dtype, mode)) return QLT_OK;
   if (plan->split_dtype == dtype && plan->split_mode == mode) return QLT_OK;
   size_t need = 0;
@@ -51,7 +52,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
                                 void *stream) {
   QLT_REQUIRE_INIT();
   QLT_CHECK_ARG(plan != nullptr, "qlt_contract_execute: null plan");
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_contract_execute: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_contract_execute: bad dtype");
   QLT_CHECK_ARG(valid_mode(mode), "qlt_contract_execute: bad mode");
   QLT_CHECK_ARG(alpha != nullptr && beta != nullptr, "qlt_contract_execute: null alpha/beta");
   QLT_CHECK_ARG(plan->device == current_device(),
@@ -59,7 +60,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
   if (plan->n_tiles == 0) return QLT_OK;
   QLT_CHECK_ARG(a != nullptr && b != nullptr && c != nullptr, "qlt_contract_execute: null data buffer");
   hipStream_t st = as_stream(stream);
-  if (mode == QLT_MODE_NATIVE) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
+  if (mode == QLT_MODE_NATIVE || dtype_is_single(dtype)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);
   if (split_profitable(plan, dtype, mode) && (plan->split_dtype != dtype || plan->split_mode != mode)) {
     QLT_CHECK_ARG(!stream_capturing(st),
                   "qlt_contract_execute: split workspace not prepared; call qlt_plan_prepare before capture");

@@ -18,6 +18,7 @@
 
 #include <cstdio>
 #include <memory>
+#include <type_traits>
 
 #include "qlt_internal.h"
 
@@ -26,6 +27,33 @@ using namespace qlt;
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Element access of the raw data: E is the stored real type (double, or
+// float for QLTEN_Float / QLTEN_ComplexFloat tensors), complex elements are
+// interleaved (re, im); arithmetic is FP64 either way.  `p` points to the
+// element (its real part).
+template <typename E, bool CPLX>
+__device__ __forceinline__ double2 ld_elem(const E *p) {
+  if constexpr (CPLX) {
+    if constexpr (sizeof(E) == 8) {
+      return *reinterpret_cast<const double2 *>(p);
+    } else {
+      const float2 f = *reinterpret_cast<const float2 *>(p);
+      return double2{static_cast<double>(f.x), static_cast<double>(f.y)};
+    }
+  } else {
+    return double2{static_cast<double>(*p), 0.0};
+  }
+}
+template <typename E, bool CPLX>
+__device__ __forceinline__ void st_elem(E *p, double re, double im) {
+  if constexpr (CPLX) {
+    if constexpr (sizeof(E) == 8) *reinterpret_cast<double2 *>(p) = double2{re, im};
+    else *reinterpret_cast<float2 *>(p) = float2{static_cast<float>(re), static_cast<float>(im)};
+  } else {
+    *p = static_cast<E>(re);
+  }
+}
 
 constexpr int KC = kKChunk;   // plan step: k-chunk of 16
 constexpr int kRealUnits = 1; // plan steps staged per pipeline step (FP64); 2 halves the occupancy and measured slower
@@ -54,11 +82,11 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 //      otherwise    -> r = tid % TM, cols tid / TM + (256 / TM) q;
 //   B' n-contiguous -> c = tid % TN, rows tid / TN + (256 / TN) q;
 //      otherwise    -> p = tid & 15, cols (tid >> 4) + 16 q.
-template <bool CPLX, int TMv, int TNv>
+template <bool CPLX, int TMv, int TNv, typename E = double>
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
-    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
+    const E *__restrict__ A, const E *__restrict__ B, E *__restrict__ C, double *__restrict__ part,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   // k extent per pipeline step: U plan steps of 16 (FP64 stages 32 k per
@@ -126,10 +154,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       const int32_t ac = (pk < kend) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
       const bool ok = (a_row[q] >= 0) && (ac >= 0);
       const int64_t e = et.a_off + a_row[q] + ac;
-      if (CPLX)
-        pa[q] = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
-      else
-        pa[q] = double2{ok ? A[e] : 0.0, 0.0};
+      pa[q] = ok ? ld_elem<E, CPLX>(A + e * P) : double2{0.0, 0.0};
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -137,10 +162,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       const int32_t br = (pk < kend) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
       const bool ok = (b_col[q] >= 0) && (br >= 0);
       const int64_t e = et.b_off + br + b_col[q];
-      if (CPLX)
-        pb[q] = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
-      else
-        pb[q] = double2{ok ? B[e] : 0.0, 0.0};
+      pb[q] = ok ? ld_elem<E, CPLX>(B + e * P) : double2{0.0, 0.0};
     }
   };
   auto stage = [&](int buf) {
@@ -220,7 +242,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
     const int bc = tid % TNv, bp0 = tid / TNv;
     constexpr int BPS = 256 / TNv;                     // B' rows between a thread's elements
     bool aok[QA], bok[QB];
-    const double *aptr[QA], *bptr[QB];
+    const E *aptr[QA], *bptr[QB];
     int64_t a_adv = 0, b_adv = 0;
 #pragma unroll
     for (int q = 0; q < QA; ++q) aok[q] = i0 + ar0 + 16 * q < seg.m;
@@ -253,15 +275,13 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 #pragma unroll
       for (int q = 0; q < QA; ++q) {
         const bool ok = aok[q] && (full || ap < kvalid);
-        if (CPLX) pa[q] = ok ? *reinterpret_cast<const double2 *>(aptr[q]) : double2{0.0, 0.0};
-        else pa[q] = double2{ok ? *aptr[q] : 0.0, 0.0};
+        pa[q] = ok ? ld_elem<E, CPLX>(aptr[q]) : double2{0.0, 0.0};
         aptr[q] += a_adv;
       }
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
         const bool ok = bok[q] && (full || bp0 + BPS * q < kvalid);
-        if (CPLX) pb[q] = ok ? *reinterpret_cast<const double2 *>(bptr[q]) : double2{0.0, 0.0};
-        else pb[q] = double2{ok ? *bptr[q] : 0.0, 0.0};
+        pb[q] = ok ? ld_elem<E, CPLX>(bptr[q]) : double2{0.0, 0.0};
         bptr[q] += b_adv;
       }
     };
@@ -383,17 +403,17 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
           if (CPLX) {
             const double xr = acc[mt][nt][v], xi = acc_i[mt][nt][v];
             double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+            E *cp = C + 2 * idx;
             if (use_beta) {
-              const double2 o = *cp;
+              const double2 o = ld_elem<E, true>(cp);
               yr += beta_r * o.x - beta_i * o.y;
               yi += beta_r * o.y + beta_i * o.x;
             }
-            *cp = double2{yr, yi};
+            st_elem<E, true>(cp, yr, yi);
           } else {
             double y = alpha_r * acc[mt][nt][v];
-            if (use_beta) y += beta_r * C[idx];
-            C[idx] = y;
+            if (use_beta) y += beta_r * static_cast<double>(C[idx]);
+            st_elem<E, false>(C + idx, y, 0.0);
           }
         }
       }
@@ -402,10 +422,10 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 // Sum of the k-slices of one output tile, in slice order, then the usual
 // epilogue.  The thread-to-element map is the MFMA kernel's, so thread tid
 // reads exactly the partial elements it would have held.
-template <bool CPLX, int TMv, int TNv>
+template <bool CPLX, int TMv, int TNv, typename E = double>
 __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restrict__ items,
                                                        const Seg *__restrict__ segs,
-                                                       const double *__restrict__ part, double *__restrict__ C,
+                                                       const double *__restrict__ part, E *__restrict__ C,
                                                        double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   constexpr int BM = TMv / 32, BN = TNv / 32;
@@ -453,17 +473,17 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restr
           const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
           if (CPLX) {
             double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            double2 *cp = reinterpret_cast<double2 *>(C) + idx;
+            E *cp = C + 2 * idx;
             if (use_beta) {
-              const double2 o = *cp;
+              const double2 o = ld_elem<E, true>(cp);
               yr += beta_r * o.x - beta_i * o.y;
               yi += beta_r * o.y + beta_i * o.x;
             }
-            *cp = double2{yr, yi};
+            st_elem<E, true>(cp, yr, yi);
           } else {
             double y = alpha_r * xr;
-            if (use_beta) y += beta_r * C[idx];
-            C[idx] = y;
+            if (use_beta) y += beta_r * static_cast<double>(C[idx]);
+            st_elem<E, false>(C + idx, y, 0.0);
           }
         }
       }
@@ -479,9 +499,9 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restr
 // C once.
 constexpr int RK = 16;            // k per stage
 
-template <bool CPLX, int NMAX, int RPT>
+template <bool CPLX, int NMAX, int RPT, typename E>
 __device__ __forceinline__ void rows_epilogue(const Seg &seg, int r0, int tid, const double (&acc)[RPT][NMAX],
-                                              const double (&acc_i)[RPT][NMAX], double *__restrict__ C,
+                                              const double (&acc_i)[RPT][NMAX], E *__restrict__ C,
                                               double alpha_r, double alpha_i, double beta_r, double beta_i) {
   const int n = seg.n;
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
@@ -496,28 +516,28 @@ __device__ __forceinline__ void rows_epilogue(const Seg &seg, int r0, int tid, c
         if (CPLX) {
           double yr = alpha_r * acc[q][j] - alpha_i * acc_i[q][j];
           double yi = alpha_r * acc_i[q][j] + alpha_i * acc[q][j];
-          double2 *cp = reinterpret_cast<double2 *>(C) + base + j;
+          E *cp = C + 2 * (base + j);
           if (use_beta) {
-            const double2 o = *cp;
+            const double2 o = ld_elem<E, true>(cp);
             yr += beta_r * o.x - beta_i * o.y;
             yi += beta_r * o.y + beta_i * o.x;
           }
-          *cp = double2{yr, yi};
+          st_elem<E, true>(cp, yr, yi);
         } else {
           double y = alpha_r * acc[q][j];
-          if (use_beta) y += beta_r * C[base + j];
-          C[base + j] = y;
+          if (use_beta) y += beta_r * static_cast<double>(C[base + j]);
+          st_elem<E, false>(C + base + j, y, 0.0);
         }
       }
     }
   }
 }
 
-template <bool CPLX, int NMAX, int RPT>
+template <bool CPLX, int NMAX, int RPT, typename E = double>
 __global__ __launch_bounds__(256) void k_rows_f64(
     const Tile *__restrict__ tiles, const Seg *__restrict__ segs, const ExecTask *__restrict__ exec,
     const BlkGather *__restrict__ ga, const BlkGather *__restrict__ gb, const GatherLayout L,
-    const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C, double *__restrict__ part,
+    const E *__restrict__ A, const E *__restrict__ B, E *__restrict__ C, double *__restrict__ part,
     double alpha_r, double alpha_i, double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   static_assert(RK * NMAX <= 256, "one B' element per thread");
@@ -578,13 +598,9 @@ __global__ __launch_bounds__(256) void k_rows_f64(
       const int32_t ro = s_brow[p], co = s_bcol[j];
       const bool ok = (ro >= 0) && (co >= 0);
       const int64_t e = et.b_off + ro + co;
-      if (CPLX) {
-        const double2 v = ok ? reinterpret_cast<const double2 *>(B)[e] : double2{0.0, 0.0};
-        Bs[0][p][j] = v.x;
-        Bs[P - 1][p][j] = v.y;
-      } else {
-        Bs[0][p][j] = ok ? B[e] : 0.0;
-      }
+      const double2 v = ok ? ld_elem<E, CPLX>(B + e * P) : double2{0.0, 0.0};
+      Bs[0][p][j] = v.x;
+      if (CPLX) Bs[P - 1][p][j] = v.y;
     }
     __syncthreads();
     const int kc = min(RK, et.k - kk);
@@ -596,14 +612,9 @@ __global__ __launch_bounds__(256) void k_rows_f64(
       for (int q = 0; q < RPT; ++q) {
         const bool ok = roff[q] >= 0;
         const int64_t e = et.a_off + roff[q] + co;
-        if (CPLX) {
-          const double2 v = ok ? reinterpret_cast<const double2 *>(A)[e] : double2{0.0, 0.0};
-          av[q] = v.x;
-          av_i[q] = v.y;
-        } else {
-          av[q] = ok ? A[e] : 0.0;
-          av_i[q] = 0.0;
-        }
+        const double2 v = ok ? ld_elem<E, CPLX>(A + e * P) : double2{0.0, 0.0};
+        av[q] = v.x;
+        av_i[q] = v.y;
       }
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
@@ -645,14 +656,14 @@ __global__ __launch_bounds__(256) void k_rows_f64(
       }
     return;
   }
-  rows_epilogue<CPLX, NMAX, RPT>(seg, r0, tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
+  rows_epilogue<CPLX, NMAX, RPT, E>(seg, r0, tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
 // Sum of the k-slices of one row tile, in slice order, then the epilogue.
-template <bool CPLX, int NMAX, int RPT>
+template <bool CPLX, int NMAX, int RPT, typename E = double>
 __global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restrict__ items,
                                                      const Seg *__restrict__ segs, const double *__restrict__ part,
-                                                     double *__restrict__ C, double alpha_r, double alpha_i,
+                                                     E *__restrict__ C, double alpha_r, double alpha_i,
                                                      double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   constexpr int SZ = 256 * RPT * NMAX * P;
@@ -680,7 +691,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restric
         }
       }
   }
-  rows_epilogue<CPLX, NMAX, RPT>(seg, it.tm * (256 * RPT), tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
+  rows_epilogue<CPLX, NMAX, RPT, E>(seg, it.tm * (256 * RPT), tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
 }  // namespace
@@ -693,7 +704,7 @@ namespace qlt {
 qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void *a, const void *b, void *c,
                                  const double *alpha, const double *beta, hipStream_t st, bool mfma_class,
                                  bool probe, ClassStreams *shared) {
-  const bool cplx = dtype == QLT_C128;
+  const bool cplx = dtype_is_complex(dtype);
   // classes with work run on the class streams when there are two or more
   int n_cls = 0;
   for (int32_t cls = mfma_class ? 0 : kNumMfmaClasses; cls < kNumTileClasses; ++cls) n_cls += plan->cls_tiles[cls] > 0;
@@ -724,97 +735,86 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
                                                         "k_grouped_gemm_f64<64x32>", "k_grouped_gemm_f64<32x32>",
                                                         "k_rows_f64<n<=4>", "k_rows_f64<n<=16>"};
     char name[64];
-    std::snprintf(name, sizeof(name), "%s %s", kNames[cls], cplx ? "c128" : "f64");
-    probe_end(ps, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], (cplx ? 16.0 : 8.0) * plan->cls_elems[cls]);
+    static const char *const kTypes[4] = {"f64", "c128", "f32", "c64"};
+    std::snprintf(name, sizeof(name), "%s %s", kNames[cls], kTypes[dtype & 3]);
+    probe_end(ps, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], static_cast<double>(dtype_bytes(dtype)) * plan->cls_elems[cls]);
   };
-  const double ar = alpha[0], ai = (dtype == QLT_C128) ? alpha[1] : 0.0;
-  const double br = beta[0], bi = (dtype == QLT_C128) ? beta[1] : 0.0;
+  const double ar = alpha[0], ai = cplx ? alpha[1] : 0.0;
+  const double br = beta[0], bi = cplx ? beta[1] : 0.0;
   const dim3 block(256);
-  const auto *A = static_cast<const double *>(a);
-  const auto *B = static_cast<const double *>(b);
-  auto *Cp = static_cast<double *>(c);
-  // MFMA classes: 64x64, 32x64, 64x32, 32x32 (tiles stored class-major)
-  int64_t base = 0, red_base = 0;
-  for (int cls = 0; cls < kNumMfmaClasses; ++cls) {
-    const int64_t cnt = plan->cls_tiles[cls];
-    const int64_t n_red = plan->cls_red[cls];
-    const Tile *tp = plan->d_tiles + base;
-    const ReduceItem *rp = plan->d_red + red_base;
-    base += cnt;
-    red_base += n_red;
-    if (!mfma_class || cnt == 0) continue;
-    const dim3 grid(static_cast<unsigned>(cnt));
-    const hipStream_t ls = stream_of(cls);
-    // this class's region of the k-slice partial sums
-    double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
-    // k-sliced tiles: the slices write partial sums, then one reduction
-    // workgroup per sliced tile adds them and applies alpha / beta
-#define QLT_LAUNCH_MFMA(CPLXV, TMV, TNV)                                                                  \
-  hipLaunchKernelGGL((k_grouped_gemm_f64<CPLXV, TMV, TNV>), grid, block, 0, ls, tp, plan->d_segs,       \
-                     plan->d_exec, plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi)
-#define QLT_LAUNCH_RED(CPLXV, TMV, TNV)                                                                   \
-  hipLaunchKernelGGL((k_reduce_slices<CPLXV, TMV, TNV>), dim3(static_cast<unsigned>(n_red)), block, 0, ls,  \
-                     rp, plan->d_segs, cls_part, Cp, ar, ai, br, bi)
-    probe_in(cls);
-    if (dtype == QLT_F64) {
-      if (cls == 0) QLT_LAUNCH_MFMA(false, 64, 64);
-      else if (cls == 1) QLT_LAUNCH_MFMA(false, 32, 64);
-      else if (cls == 2) QLT_LAUNCH_MFMA(false, 64, 32);
-      else QLT_LAUNCH_MFMA(false, 32, 32);
-    } else {
-      if (cls == 0) QLT_LAUNCH_MFMA(true, 64, 64);
-      else if (cls == 1) QLT_LAUNCH_MFMA(true, 32, 64);
-      else if (cls == 2) QLT_LAUNCH_MFMA(true, 64, 32);
-      else QLT_LAUNCH_MFMA(true, 32, 32);
+  // every kernel instance of one element storage type E (double, or float
+  // for the single-precision element types) and complexity
+  auto run = [&](auto e_tag, auto cplx_tag) {
+    using E = decltype(e_tag);
+    constexpr bool CX = decltype(cplx_tag)::value;
+    const auto *A = static_cast<const E *>(a);
+    const auto *B = static_cast<const E *>(b);
+    auto *Cp = static_cast<E *>(c);
+    // MFMA classes: 64x64, 32x64, 64x32, 32x32 (tiles stored class-major)
+    int64_t base = 0, red_base = 0;
+    for (int cls = 0; cls < kNumMfmaClasses; ++cls) {
+      const int64_t cnt = plan->cls_tiles[cls];
+      const int64_t n_red = plan->cls_red[cls];
+      const Tile *tp = plan->d_tiles + base;
+      const ReduceItem *rp = plan->d_red + red_base;
+      base += cnt;
+      red_base += n_red;
+      if (!mfma_class || cnt == 0) continue;
+      const dim3 grid(static_cast<unsigned>(cnt));
+      const hipStream_t ls = stream_of(cls);
+      // this class's region of the k-slice partial sums
+      double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
+      // k-sliced tiles: the slices write partial sums, then one reduction
+      // workgroup per sliced tile adds them and applies alpha / beta
+      auto mfma = [&](auto tm_tag, auto tn_tag) {
+        constexpr int TMV = decltype(tm_tag)::value, TNV = decltype(tn_tag)::value;
+        probe_in(cls);
+        hipLaunchKernelGGL((k_grouped_gemm_f64<CX, TMV, TNV, E>), grid, block, 0, ls, tp, plan->d_segs, plan->d_exec,
+                           plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi);
+        probe_out(cls);
+        if (n_red > 0)
+          hipLaunchKernelGGL((k_reduce_slices<CX, TMV, TNV, E>), dim3(static_cast<unsigned>(n_red)), block, 0, ls, rp,
+                             plan->d_segs, cls_part, Cp, ar, ai, br, bi);
+      };
+      using I32 = std::integral_constant<int, 32>;
+      using I64 = std::integral_constant<int, 64>;
+      if (cls == 0) mfma(I64{}, I64{});
+      else if (cls == 1) mfma(I32{}, I64{});
+      else if (cls == 2) mfma(I64{}, I32{});
+      else mfma(I32{}, I32{});
     }
-    probe_out(cls);
-    if (n_red > 0) {
-      if (dtype == QLT_F64) {
-        if (cls == 0) QLT_LAUNCH_RED(false, 64, 64);
-        else if (cls == 1) QLT_LAUNCH_RED(false, 32, 64);
-        else if (cls == 2) QLT_LAUNCH_RED(false, 64, 32);
-        else QLT_LAUNCH_RED(false, 32, 32);
-      } else {
-        if (cls == 0) QLT_LAUNCH_RED(true, 64, 64);
-        else if (cls == 1) QLT_LAUNCH_RED(true, 32, 64);
-        else if (cls == 2) QLT_LAUNCH_RED(true, 64, 32);
-        else QLT_LAUNCH_RED(true, 32, 32);
-      }
+    // row classes (n <= 4, n <= 16), each followed by its k-slice reduction
+    for (int32_t cls = kTileRows4; cls <= kTileRows16; ++cls) {
+      const int64_t cnt = plan->cls_tiles[cls];
+      const int64_t n_red = plan->cls_red[cls];
+      const Tile *tp = plan->d_tiles + base;
+      const ReduceItem *rp = plan->d_red + red_base;
+      base += cnt;
+      red_base += n_red;
+      if (cnt == 0) continue;
+      const dim3 grid(static_cast<unsigned>(cnt)), rgrid(static_cast<unsigned>(n_red > 0 ? n_red : 1));
+      const hipStream_t ls = stream_of(cls);
+      // this class's region of the k-slice partial sums
+      double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
+      auto rows = [&](auto nmax_tag, auto rpt_tag) {
+        constexpr int NMAXV = decltype(nmax_tag)::value, RPTV = decltype(rpt_tag)::value;
+        probe_in(cls);
+        hipLaunchKernelGGL((k_rows_f64<CX, NMAXV, RPTV, E>), grid, block, 0, ls, tp, plan->d_segs, plan->d_exec,
+                           plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi);
+        probe_out(cls);
+        if (n_red > 0)
+          hipLaunchKernelGGL((k_reduce_rows<CX, NMAXV, RPTV, E>), rgrid, block, 0, ls, rp, plan->d_segs, cls_part, Cp,
+                             ar, ai, br, bi);
+      };
+      if (cls == kTileRows4) rows(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
+      else rows(std::integral_constant<int, 16>{}, std::integral_constant<int, 1>{});
     }
-#undef QLT_LAUNCH_RED
-#undef QLT_LAUNCH_MFMA
-  }
-  // row classes (n <= 4, n <= 16), each followed by its k-slice reduction
-  for (int32_t cls = kTileRows4; cls <= kTileRows16; ++cls) {
-    const int64_t cnt = plan->cls_tiles[cls];
-    const int64_t n_red = plan->cls_red[cls];
-    const Tile *tp = plan->d_tiles + base;
-    const ReduceItem *rp = plan->d_red + red_base;
-    base += cnt;
-    red_base += n_red;
-    if (cnt == 0) continue;
-    const dim3 grid(static_cast<unsigned>(cnt)), rgrid(static_cast<unsigned>(n_red > 0 ? n_red : 1));
-    const hipStream_t ls = stream_of(cls);
-    // this class's region of the k-slice partial sums
-    double *const cls_part = plan->d_part != nullptr ? plan->d_part + plan->cls_part_off[cls] : nullptr;
-#define QLT_LAUNCH_ROWS(CPLXV, NMAXV, RPTV)                                                                     \
-  do {                                                                                                          \
-    hipLaunchKernelGGL((k_rows_f64<CPLXV, NMAXV, RPTV>), grid, block, 0, ls, tp, plan->d_segs, plan->d_exec,   \
-                       plan->d_ga, plan->d_gb, plan->layout, A, B, Cp, cls_part, ar, ai, br, bi);          \
-    probe_out(cls);                                                                                             \
-    if (n_red > 0)                                                                                              \
-      hipLaunchKernelGGL((k_reduce_rows<CPLXV, NMAXV, RPTV>), rgrid, block, 0, ls, rp, plan->d_segs,           \
-                         cls_part, Cp, ar, ai, br, bi);                                                     \
-  } while (0)
-    probe_in(cls);
-    if (cls == kTileRows4) {
-      if (dtype == QLT_F64) QLT_LAUNCH_ROWS(false, 4, 4);
-      else QLT_LAUNCH_ROWS(true, 4, 4);
-    } else {
-      if (dtype == QLT_F64) QLT_LAUNCH_ROWS(false, 16, 1);
-      else QLT_LAUNCH_ROWS(true, 16, 1);
-    }
-#undef QLT_LAUNCH_ROWS
+  };
+  switch (dtype) {
+    case QLT_F64: run(double{}, std::false_type{}); break;
+    case QLT_C128: run(double{}, std::true_type{}); break;
+    case QLT_F32: run(float{}, std::false_type{}); break;
+    default: run(float{}, std::true_type{}); break;
   }
   QLT_HIP(hipGetLastError());
   return QLT_OK;

@@ -144,6 +144,12 @@ constexpr int32_t kNumTileClasses = 6;
 __host__ __device__ constexpr int32_t mfma_tile_m(int32_t cls) { return (cls == 1 || cls == 3) ? 32 : 64; }
 __host__ __device__ constexpr int32_t mfma_tile_n(int32_t cls) { return (cls == 2 || cls == 3) ? 32 : 64; }
 __host__ __device__ constexpr bool is_mfma_class(int32_t cls) { return cls < kNumMfmaClasses; }
+// Element types: FP64 / complex128, and FP32 / complex64 (stored in single
+// precision, contracted in FP64 arithmetic).
+constexpr bool valid_dtype(int32_t d) { return d == QLT_F64 || d == QLT_C128 || d == QLT_F32 || d == QLT_C64; }
+constexpr bool dtype_is_complex(int32_t d) { return d == QLT_C128 || d == QLT_C64; }
+constexpr bool dtype_is_single(int32_t d) { return d == QLT_F32 || d == QLT_C64; }
+constexpr int32_t dtype_bytes(int32_t d) { return d == QLT_C128 ? 16 : (d == QLT_F32 ? 4 : 8); }
 constexpr bool valid_mode(int32_t mode) {
   return mode == QLT_MODE_NATIVE || mode == QLT_MODE_SPLIT_BF16 || mode == QLT_MODE_SPLIT_I8;
 }

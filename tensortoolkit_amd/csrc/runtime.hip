@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -179,14 +180,18 @@ __global__ void k_zero(uint8_t *p, int64_t bytes) {
   for (int64_t j = nwords * 8 + i; j < bytes; j += stride) p[j] = 0;
 }
 
-__global__ void k_axpby_real(int64_t n, double a, const double *x, double b, double *y) {
+// y = a x + b y over n elements stored as E (double or float), FP64 arithmetic.
+template <typename E>
+__global__ void k_axpby_real(int64_t n, double a, const E *x, double b, E *y) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
-    y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    const double xv = static_cast<double>(x[i]);
+    y[i] = static_cast<E>((b == 0.0) ? a * xv : a * xv + b * static_cast<double>(y[i]));
+  }
 }
 
-__global__ void k_axpby_cplx(int64_t n, double ar, double ai, const double *x, double br,
-                             double bi, double *y) {
+template <typename E>
+__global__ void k_axpby_cplx(int64_t n, double ar, double ai, const E *x, double br, double bi, E *y) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   const bool bzero = (br == 0.0 && bi == 0.0);
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
@@ -197,15 +202,14 @@ __global__ void k_axpby_cplx(int64_t n, double ar, double ai, const double *x, d
       yr += br * vr - bi * vi;
       yi += br * vi + bi * vr;
     }
-    y[2 * i] = yr;
-    y[2 * i + 1] = yi;
+    y[2 * i] = static_cast<E>(yr);
+    y[2 * i + 1] = static_cast<E>(yi);
   }
 }
 
 // Two-level reduction of sum conj(x) * y: block partials, then one block.
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double *x, const double *y,
-                                                     double *partial) {
+template <bool CPLX, typename E = double>
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const E *x, const E *y, double *partial) {
   __shared__ double sr[256], si[256];
   double accr = 0.0, acci = 0.0;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double *x,
       accr += xr * yr + xi * yi;
       acci += xr * yi - xi * yr;
     } else {
-      accr += x[i] * y[i];
+      accr += static_cast<double>(x[i]) * static_cast<double>(y[i]);
     }
   }
   sr[threadIdx.x] = accr;
@@ -265,27 +269,66 @@ __global__ void k_real_to_complex(int64_t n, const double *x, double *z) {
   }
 }
 
-__global__ void k_conjugate(int64_t n, double *z) {
+template <typename E>
+__global__ void k_conjugate(int64_t n, E *z) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
     z[2 * i + 1] = -z[2 * i + 1];
 }
 
+// Element conversion between storage types: FS/FD the source/destination
+// real types, CS/CD whether the element is complex (real -> complex sets the
+// imaginary part to 0; complex -> real is not offered).
+template <typename FS, bool CS, typename FD, bool CD>
+__global__ void k_convert(int64_t n, const FS *src, FD *dst) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    if constexpr (CD) {
+      dst[2 * i] = static_cast<FD>(CS ? src[2 * i] : src[i]);
+      dst[2 * i + 1] = CS ? static_cast<FD>(src[2 * i + 1]) : FD(0);
+    } else {
+      dst[i] = static_cast<FD>(src[i]);
+    }
+  }
+}
+
+// The seeded uniform fill of qlt_fill_uniform, rounded to float.
+__global__ void k_fill_uniform_f32(float *x, int64_t n, uint64_t seed, int64_t first, double lo, double width) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    x[i] = static_cast<float>(lo + width * splitmix_unit(seed, static_cast<uint64_t>(first + i)));
+}
+
 // One block per segment; scale by a real or complex factor.
-template <bool CPLX>
-__global__ void k_scale_segments(const int64_t *offs, const int64_t *lens, double sr, double si,
-                                 double *x) {
+template <bool CPLX, typename E = double>
+__global__ void k_scale_segments(const int64_t *offs, const int64_t *lens, double sr, double si, E *x) {
   const int64_t off = offs[blockIdx.x], len = lens[blockIdx.x];
   for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
     if (CPLX) {
-      double *p = x + 2 * (off + i);
+      E *p = x + 2 * (off + i);
       const double r = p[0], im = p[1];
-      p[0] = sr * r - si * im;
-      p[1] = sr * im + si * r;
+      p[0] = static_cast<E>(sr * r - si * im);
+      p[1] = static_cast<E>(sr * im + si * r);
     } else {
-      x[off + i] *= sr;
+      x[off + i] = static_cast<E>(sr * static_cast<double>(x[off + i]));
     }
   }
+}
+
+void launch_scale_segments(int32_t dtype, unsigned n_segs, const int64_t *offs, const int64_t *lens, const double *s,
+                           void *x, hipStream_t st) {
+  if (dtype == QLT_F64)
+    hipLaunchKernelGGL((k_scale_segments<false, double>), dim3(n_segs), dim3(256), 0, st, offs, lens, s[0], 0.0,
+                       static_cast<double *>(x));
+  else if (dtype == QLT_C128)
+    hipLaunchKernelGGL((k_scale_segments<true, double>), dim3(n_segs), dim3(256), 0, st, offs, lens, s[0], s[1],
+                       static_cast<double *>(x));
+  else if (dtype == QLT_F32)
+    hipLaunchKernelGGL((k_scale_segments<false, float>), dim3(n_segs), dim3(256), 0, st, offs, lens, s[0], 0.0,
+                       static_cast<float *>(x));
+  else
+    hipLaunchKernelGGL((k_scale_segments<true, float>), dim3(n_segs), dim3(256), 0, st, offs, lens, s[0], s[1],
+                       static_cast<float *>(x));
 }
 
 inline unsigned grid_for(int64_t n, int threads = 256, int64_t cap = 8192) {
@@ -509,15 +552,23 @@ qlt_status qlt_fill_uniform(void *x, int64_t n_doubles, uint64_t seed, int64_t f
 qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x, const double *b,
                      void *y, void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_axpby: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_axpby: bad dtype");
   QLT_CHECK_ARG(a != nullptr && b != nullptr, "qlt_axpby: null scalar");
   if (n <= 0) return QLT_OK;
+  const dim3 g(grid_for(n));
+  hipStream_t st = as_stream(stream);
   if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_axpby_real, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, a[0],
-                       static_cast<const double *>(x), b[0], static_cast<double *>(y));
+    hipLaunchKernelGGL(k_axpby_real<double>, g, dim3(256), 0, st, n, a[0], static_cast<const double *>(x), b[0],
+                       static_cast<double *>(y));
+  else if (dtype == QLT_C128)
+    hipLaunchKernelGGL(k_axpby_cplx<double>, g, dim3(256), 0, st, n, a[0], a[1], static_cast<const double *>(x), b[0],
+                       b[1], static_cast<double *>(y));
+  else if (dtype == QLT_F32)
+    hipLaunchKernelGGL(k_axpby_real<float>, g, dim3(256), 0, st, n, a[0], static_cast<const float *>(x), b[0],
+                       static_cast<float *>(y));
   else
-    hipLaunchKernelGGL(k_axpby_cplx, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, a[0],
-                       a[1], static_cast<const double *>(x), b[0], b[1], static_cast<double *>(y));
+    hipLaunchKernelGGL(k_axpby_cplx<float>, g, dim3(256), 0, st, n, a[0], a[1], static_cast<const float *>(x), b[0],
+                       b[1], static_cast<float *>(y));
   QLT_HIP(hipGetLastError());
   return QLT_OK;
 }
@@ -525,18 +576,24 @@ qlt_status qlt_axpby(int32_t dtype, int64_t n, const double *a, const void *x, c
 qlt_status qlt_dot(int32_t dtype, int64_t n, const void *x, const void *y, double *out,
                    void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_dot: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_dot: bad dtype");
   QLT_CHECK_ARG(out != nullptr, "qlt_dot: null output");
   out[0] = out[1] = 0.0;
   if (n <= 0) return QLT_OK;
   hipStream_t s = as_stream(stream);
   const unsigned g = std::min<unsigned>(grid_for(n), kReduceBlocks);
   if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_dot_partial<false>, dim3(g), dim3(256), 0, s, n,
+    hipLaunchKernelGGL((k_dot_partial<false, double>), dim3(g), dim3(256), 0, s, n,
                        static_cast<const double *>(x), static_cast<const double *>(y), g_reduce_buf);
+  else if (dtype == QLT_C128)
+    hipLaunchKernelGGL((k_dot_partial<true, double>), dim3(g), dim3(256), 0, s, n,
+                       static_cast<const double *>(x), static_cast<const double *>(y), g_reduce_buf);
+  else if (dtype == QLT_F32)
+    hipLaunchKernelGGL((k_dot_partial<false, float>), dim3(g), dim3(256), 0, s, n,
+                       static_cast<const float *>(x), static_cast<const float *>(y), g_reduce_buf);
   else
-    hipLaunchKernelGGL(k_dot_partial<true>, dim3(g), dim3(256), 0, s, n,
-                       static_cast<const double *>(x), static_cast<const double *>(y), g_reduce_buf);
+    hipLaunchKernelGGL((k_dot_partial<true, float>), dim3(g), dim3(256), 0, s, n,
+                       static_cast<const float *>(x), static_cast<const float *>(y), g_reduce_buf);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, static_cast<int>(g), g_reduce_buf);
   QLT_HIP(hipGetLastError());
   QLT_HIP(hipMemcpyAsync(out, g_reduce_buf + 2 * kReduceBlocks, 2 * sizeof(double),
@@ -558,7 +615,64 @@ qlt_status qlt_conjugate(int64_t n, double *z, void *stream) {
   QLT_REQUIRE_INIT();
   if (n <= 0) return QLT_OK;
   QLT_CHECK_ARG(z != nullptr, "qlt_conjugate: null buffer");
-  hipLaunchKernelGGL(k_conjugate, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, z);
+  hipLaunchKernelGGL(k_conjugate<double>, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, z);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_conjugate_c64(int64_t n, float *z, void *stream) {
+  QLT_REQUIRE_INIT();
+  if (n <= 0) return QLT_OK;
+  QLT_CHECK_ARG(z != nullptr, "qlt_conjugate_c64: null buffer");
+  hipLaunchKernelGGL(k_conjugate<float>, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), n, z);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_fill_uniform_f32(void *x, int64_t n_floats, uint64_t seed, int64_t first_index, double lo,
+                                double hi, void *stream) {
+  QLT_REQUIRE_INIT();
+  if (n_floats <= 0) return QLT_OK;
+  QLT_CHECK_ARG(x != nullptr, "qlt_fill_uniform_f32: null buffer");
+  hipLaunchKernelGGL(k_fill_uniform_f32, dim3(grid_for(n_floats)), dim3(256), 0, as_stream(stream),
+                     static_cast<float *>(x), n_floats, seed, first_index, lo, hi - lo);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_convert(int32_t src_dtype, int32_t dst_dtype, int64_t n, const void *src, void *dst, void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_CHECK_ARG(valid_dtype(src_dtype) && valid_dtype(dst_dtype), "qlt_convert: bad dtype");
+  QLT_CHECK_ARG(!dtype_is_complex(src_dtype) || dtype_is_complex(dst_dtype),
+                "qlt_convert: complex to real is not a conversion");
+  if (n <= 0) return QLT_OK;
+  QLT_CHECK_ARG(src != nullptr && dst != nullptr, "qlt_convert: null buffer");
+  const dim3 g(grid_for(n));
+  hipStream_t st = as_stream(stream);
+  const bool cs = dtype_is_complex(src_dtype), cd = dtype_is_complex(dst_dtype);
+  const bool ss = dtype_is_single(src_dtype), sd = dtype_is_single(dst_dtype);
+  auto launch = [&](auto fs, auto cs_tag, auto fd, auto cd_tag) {
+    using FS = decltype(fs);
+    using FD = decltype(fd);
+    hipLaunchKernelGGL((k_convert<FS, decltype(cs_tag)::value, FD, decltype(cd_tag)::value>), g, dim3(256), 0, st, n,
+                       static_cast<const FS *>(src), static_cast<FD *>(dst));
+  };
+  auto by_dst = [&](auto fs, auto cs_tag) {
+    if (cd) {
+      if (sd) launch(fs, cs_tag, float{}, std::true_type{});
+      else launch(fs, cs_tag, double{}, std::true_type{});
+    } else {
+      if (sd) launch(fs, cs_tag, float{}, std::false_type{});
+      else launch(fs, cs_tag, double{}, std::false_type{});
+    }
+  };
+  if (cs) {
+    if (ss) by_dst(float{}, std::true_type{});
+    else by_dst(double{}, std::true_type{});
+  } else {
+    if (ss) by_dst(float{}, std::false_type{});
+    else by_dst(double{}, std::false_type{});
+  }
   QLT_HIP(hipGetLastError());
   return QLT_OK;
 }
@@ -571,9 +685,10 @@ struct Copy2D {
   int64_t rows, cols, src_off, src_ld, dst_off, dst_ld;
 };
 
-// One workgroup per rectangle; W doubles per element.
-template <int W>
-__global__ void k_copy_2d(const Copy2D *items, const double *src, double *dst) {
+// One workgroup per rectangle; W words of type T per element (one double
+// for F64 and C64, two for C128, one float for F32).
+template <int W, typename T = double>
+__global__ void k_copy_2d(const Copy2D *items, const T *src, T *dst) {
   const Copy2D it = items[blockIdx.x];
   const int64_t n = it.rows * it.cols;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
@@ -592,7 +707,7 @@ qlt_status qlt_copy_blocks_2d(int32_t dtype, int64_t n, const int64_t *rows, con
                               const void *src, const int64_t *src_offsets, const int64_t *src_ld,
                               void *dst, const int64_t *dst_offsets, const int64_t *dst_ld, void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_copy_blocks_2d: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_copy_blocks_2d: bad dtype");
   QLT_CHECK_ARG(n >= 0 && n < (int64_t(1) << 31), "qlt_copy_blocks_2d: bad rectangle count");
   if (n == 0) return QLT_OK;
   QLT_CHECK_ARG(rows && cols && src_offsets && src_ld && dst_offsets && dst_ld, "qlt_copy_blocks_2d: null array");
@@ -609,12 +724,15 @@ qlt_status qlt_copy_blocks_2d(int32_t dtype, int64_t n, const int64_t *rows, con
   Copy2D *d = nullptr;
   QLT_HIP(hipMallocAsync(reinterpret_cast<void **>(&d), sizeof(Copy2D) * h.size(), st));
   QLT_HIP(hipMemcpyAsync(d, h.data(), sizeof(Copy2D) * h.size(), hipMemcpyHostToDevice, st));
-  if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_copy_2d<1>, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
+  if (dtype == QLT_F64 || dtype == QLT_C64)
+    hipLaunchKernelGGL((k_copy_2d<1, double>), dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
+                       static_cast<const double *>(src), static_cast<double *>(dst));
+  else if (dtype == QLT_C128)
+    hipLaunchKernelGGL((k_copy_2d<2, double>), dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
                        static_cast<const double *>(src), static_cast<double *>(dst));
   else
-    hipLaunchKernelGGL(k_copy_2d<2>, dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
-                       static_cast<const double *>(src), static_cast<double *>(dst));
+    hipLaunchKernelGGL((k_copy_2d<1, float>), dim3(static_cast<unsigned>(n)), dim3(256), 0, st, d,
+                       static_cast<const float *>(src), static_cast<float *>(dst));
   QLT_HIP(hipGetLastError());
   QLT_HIP(hipStreamSynchronize(st));
   QLT_HIP(hipFreeAsync(d, st));
@@ -625,7 +743,7 @@ qlt_status qlt_copy_blocks_2d(int32_t dtype, int64_t n, const int64_t *rows, con
 qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offsets,
                               const int64_t *lengths, const double *s, void *x, void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_scale_segments: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_scale_segments: bad dtype");
   QLT_CHECK_ARG(s != nullptr, "qlt_scale_segments: null scalar");
   if (n_segs <= 0) return QLT_OK;
   QLT_CHECK_ARG(offsets != nullptr && lengths != nullptr && x != nullptr,
@@ -636,12 +754,7 @@ qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offs
   QLT_HIP(hipMallocAsync(reinterpret_cast<void **>(&d), 2 * n_segs * sizeof(int64_t), st));
   QLT_HIP(hipMemcpyAsync(d, offsets, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
   QLT_HIP(hipMemcpyAsync(d + n_segs, lengths, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_scale_segments<false>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
-                       d, d + n_segs, s[0], 0.0, static_cast<double *>(x));
-  else
-    hipLaunchKernelGGL(k_scale_segments<true>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
-                       d, d + n_segs, s[0], s[1], static_cast<double *>(x));
+  launch_scale_segments(dtype, static_cast<unsigned>(n_segs), d, d + n_segs, s, x, st);
   QLT_HIP(hipGetLastError());
   QLT_HIP(hipStreamSynchronize(st));
   QLT_HIP(hipFreeAsync(d, st));
@@ -652,7 +765,7 @@ qlt_status qlt_scale_segments(int32_t dtype, int64_t n_segs, const int64_t *offs
 qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64_t *offsets,
                                       const int64_t *lengths, const double *s, void *x, void *stream) {
   QLT_REQUIRE_INIT();
-  QLT_CHECK_ARG(dtype == QLT_F64 || dtype == QLT_C128, "qlt_scale_segments_noalloc: bad dtype");
+  QLT_CHECK_ARG(valid_dtype(dtype), "qlt_scale_segments_noalloc: bad dtype");
   QLT_CHECK_ARG(s != nullptr, "qlt_scale_segments_noalloc: null scalar");
   if (n_segs <= 0) return QLT_OK;
   QLT_CHECK_ARG(n_segs <= kNoAllocMaxSegs,
@@ -671,12 +784,7 @@ qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64
   int64_t *d = g_scratch_ptr[dev];
   QLT_HIP(hipMemcpyAsync(d, offsets, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
   QLT_HIP(hipMemcpyAsync(d + kNoAllocMaxSegs, lengths, n_segs * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  if (dtype == QLT_F64)
-    hipLaunchKernelGGL(k_scale_segments<false>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
-                       d, d + kNoAllocMaxSegs, s[0], 0.0, static_cast<double *>(x));
-  else
-    hipLaunchKernelGGL(k_scale_segments<true>, dim3(static_cast<unsigned>(n_segs)), dim3(256), 0, st,
-                       d, d + kNoAllocMaxSegs, s[0], s[1], static_cast<double *>(x));
+  launch_scale_segments(dtype, static_cast<unsigned>(n_segs), d, d + kNoAllocMaxSegs, s, x, st);
   QLT_HIP(hipGetLastError());
   // the segment list is reused by the next call: return only once this one is done
   QLT_HIP(hipStreamSynchronize(st));

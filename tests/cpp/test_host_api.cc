@@ -18,6 +18,7 @@
 #include <numeric>
 #include <set>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qlten/qlten.h"
@@ -212,7 +213,7 @@ void CheckMatBasedContract(const QLTensor<T, U1QN> &ta, const std::vector<size_t
     CHECK(got.size() == ref.size(), nm + ": size");
     if (got.size() == ref.size()) {
       const double err = RelDiff(got, ref);
-      CHECK(err < 1e-13, nm + ": rel err " + std::to_string(err));
+      CHECK(err < (IsSingle<T>() ? 1e-5 : 1e-13), nm + ": rel err " + std::to_string(err));
     }
     if (i > 0) {
       CHECK(c.GetBlkSparDataTen().GetActualRawDataSize() == res[0].second.GetBlkSparDataTen().GetActualRawDataSize(),
@@ -531,6 +532,93 @@ int main() {
     };
     cases(QLTEN_Double(0), "real");
     cases(QLTEN_Complex(0), "complex");
+    // the reference runs the same cases on QLTEN_Float / QLTEN_ComplexFloat
+    // tensors (test_ten_ctrct.cc:798-859)
+    cases(QLTEN_Float(0), "float");
+    cases(QLTEN_ComplexFloat(0), "complex float");
+  }
+
+  // ---- single-precision element types (value_t.h:38-41): stored as float,
+  // contracted with FP64 arithmetic and rounded once, so the result is the
+  // FP64 contraction of the float inputs to within float rounding ----
+  {
+    auto widen = [](const auto &v) {
+      using T = typename std::decay_t<decltype(v)>::value_type;
+      using W = std::conditional_t<IsComplex<T>(), QLTEN_Complex, QLTEN_Double>;
+      std::vector<W> out(v.size());
+      for (size_t i = 0; i < v.size(); ++i) out[i] = W(v[i]);
+      return out;
+    };
+    auto check_single = [&](auto tag, const std::string &name) {
+      using T = decltype(tag);
+      using Ten = QLTensor<T, U1QN>;
+      Ten a({f.idx_in_l, f.idx_out_s, f.idx_out_l});
+      Ten b({f.idx_in_l, f.idx_in_s, f.idx_out_s});
+      a.Random(U1QN(0));
+      b.Random(U1QN(1));
+      for (const auto &ax : std::vector<std::pair<std::vector<size_t>, std::vector<size_t>>>{
+               {{2}, {0}}, {{2, 1}, {0, 1}}}) {
+        Ten c;
+        Contract(&a, &b, {ax.first, ax.second}, &c);
+        std::vector<size_t> a_sv, b_sv;
+        for (size_t i = 0; i < a.Rank(); ++i)
+          if (std::find(ax.first.begin(), ax.first.end(), i) == ax.first.end()) a_sv.push_back(i);
+        for (size_t i = 0; i < b.Rank(); ++i)
+          if (std::find(ax.second.begin(), ax.second.end(), i) == ax.second.end()) b_sv.push_back(i);
+        const auto ref = DenseContract(widen(Densify(a)), a.GetShape(), widen(Densify(b)), b.GetShape(), ax.first,
+                                       ax.second, a_sv, b_sv);
+        const double err = RelDiff(widen(Densify(c)), ref);
+        CHECK(err < 1e-7, name + " contraction vs FP64 of the float inputs: rel err " + std::to_string(err));
+      }
+      const auto n2 = a.Get2Norm();
+      double ref2 = 0;
+      for (auto v : widen(Densify(a))) ref2 += std::norm(std::complex<double>(v));
+      CHECK(std::abs(static_cast<double>(n2) - std::sqrt(ref2)) < 1e-6 * std::sqrt(ref2), name + " Get2Norm");
+      Ten a2 = a * T(2);
+      CHECK(RelDiff(widen(Densify(a2)), [&] { auto d = widen(Densify(a)); for (auto &v : d) v *= 2.0; return d; }()) ==
+                0.0,
+            name + " operator*");
+    };
+    check_single(QLTEN_Float(0), "float");
+    check_single(QLTEN_ComplexFloat(0), "complex float");
+    // mixed float x complex float (ten_ctrct.h:322-350) equals the promoted contraction
+    QLTensor<QLTEN_Float, U1QN> ar({f.idx_in_l, f.idx_out_s, f.idx_out_l});
+    QLTensor<QLTEN_ComplexFloat, U1QN> bz({f.idx_in_l, f.idx_in_s, f.idx_out_s});
+    ar.Random(U1QN(0));
+    bz.Random(U1QN(1));
+    QLTensor<QLTEN_ComplexFloat, U1QN> c_mixed, c_promoted, c_mixed2, c_promoted2;
+    Contract(&ar, &bz, {{2}, {0}}, &c_mixed);
+    auto arz = ToComplex(ar);
+    Contract(&arz, &bz, {{2}, {0}}, &c_promoted);
+    CHECK(RelDiff(widen(Densify(c_mixed)), widen(Densify(c_promoted))) == 0.0, "mixed float x complex float");
+    Contract(&bz, &ar, {{0}, {2}}, &c_mixed2);
+    Contract(&bz, &arz, {{0}, {2}}, &c_promoted2);
+    CHECK(RelDiff(widen(Densify(c_mixed2)), widen(Densify(c_promoted2))) == 0.0, "mixed complex float x float");
+    // t . dag(t) = sum |t|^2 for complex float
+    QLTensor<QLTEN_ComplexFloat, U1QN> t({f.idx_out_l});
+    t.Random(U1QN(0));
+    auto td = Dag(t);
+    QLTensor<QLTEN_ComplexFloat, U1QN> sc;
+    Contract(&t, &td, {{0}, {0}}, &sc);
+    double ref = 0;
+    for (auto v : widen(Densify(t))) ref += std::norm(v);
+    CHECK(std::abs(std::complex<double>(sc.GetElem({})) - std::complex<double>(ref, 0.0)) < 1e-6 * ref,
+          "complex float t.dag(t)");
+    // block SVD of a float tensor: the known answer in single precision
+    QNSectorVec<U1QN> one = {QNSctT(U1QN(0), 2)};
+    QLTensor<QLTEN_Float, U1QN> m({U1Idx(one, IN), U1Idx(one, OUT)});
+    m.Fill(U1QN(0), 0.0f);
+    m.SetElem({0, 0}, 1.0f);
+    m.SetElem({0, 1}, 2.0f);
+    m.SetElem({1, 0}, 3.0f);
+    m.SetElem({1, 1}, 4.0f);
+    QLTensor<QLTEN_Float, U1QN> mu, mvt, ms;
+    float merr = -1;
+    size_t mD = 0;
+    SVD(&m, 1, U1QN(0), 0.0f, 1, 10, &mu, &ms, &mvt, &merr, &mD);
+    CHECK(mD == 2, "float svd D");
+    CHECK(std::abs(ms.GetElem({0, 0}) - 5.464985704219043f) < 1e-6f * 5.5f, "float svd sigma_0");
+    CHECK(std::abs(ms.GetElem({1, 1}) - 0.365966190626257f) < 1e-6f * 5.5f, "float svd sigma_1");
   }
 
   // ---- accumulate: C = beta C + alpha A.B (contract_contiguous_axes.h:664-782) ----

@@ -45,7 +45,7 @@ def declared_constants():
 
 def test_binding_constants_match_the_header():
     consts = declared_constants()
-    for name in ("QLT_OK", "QLT_F64", "QLT_C128", "QLT_MODE_NATIVE", "QLT_MODE_SPLIT_BF16",
+    for name in ("QLT_OK", "QLT_F64", "QLT_C128", "QLT_F32", "QLT_C64", "QLT_MODE_NATIVE", "QLT_MODE_SPLIT_BF16",
                  "QLT_MODE_SPLIT_I8", "QLT_MAX_RANK", "QLT_TUNE_BUCKETS", "QLT_TILE_CLASSES"):
         assert name in consts, name
     for name, val in consts.items():
