@@ -110,9 +110,10 @@ typedef struct {
   int32_t c_is_scalar;   /* 1 if every axis was contracted                         */
   int64_t sum_mkn;       /* sum over tasks of m*k*n (flops = 2x real, 8x complex;
                             tensor_op_cost.h:384-406)                             */
-  int64_t n_tiles;       /* output tiles of the grouped kernel                     */
+  int64_t n_tiles;       /* output tiles of the grouped kernels (row panels aside)  */
   int64_t a_unique_elems;/* elements of A blocks touched by at least one task      */
   int64_t b_unique_elems;/* elements of B blocks touched by at least one task      */
+  int64_t n_panel_tiles; /* 1024-row tiles of the row panels (qlt_set_row_panels)  */
 } qlt_plan_info;
 
 /* ---- runtime ---------------------------------------------------------- */
@@ -192,6 +193,17 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
  * (framework/hp_numeric/omp_set.h:67-83); it has no reference counterpart. */
 qlt_status qlt_set_split_min_mean_k(double min_mean_k);
 double qlt_get_split_min_mean_k(void);
+
+/* Row panels (default on, 1), read when a plan is built: the narrow output
+ * blocks (n <= 4) that share their coordinates on A's saved axes, and
+ * together read at most 4 contracted columns of A', run as one unit that
+ * reads each A element once for all of them.  0 keeps every such block on
+ * the per-block row kernel.  The values are bitwise the same either way
+ * (same sums in the same order).  An execution-strategy knob with no
+ * reference counterpart (the reference runs one BLAS call per task,
+ * global_operations.h:919-978). */
+qlt_status qlt_set_row_panels(int32_t enable);
+int32_t qlt_get_row_panels(void);
 
 /* ---- tile-shape and split-k tuning; no counterpart in the reference (its
  * per-task BLAS calls have no tiles) ----

@@ -92,6 +92,7 @@ class PlanInfo(C.Structure):
         ("n_tiles", C.c_int64),
         ("a_unique_elems", C.c_int64),
         ("b_unique_elems", C.c_int64),
+        ("n_panel_tiles", C.c_int64),
     ]
 
 
@@ -134,6 +135,8 @@ _SIGS = {
     "qlt_autotune": ([C.c_int32, C.POINTER(TuneItem), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                       C.POINTER(Tuning), C.POINTER(C.c_double), C.POINTER(C.c_double), _i32p], C.c_int32),
     "qlt_get_split_min_mean_k": ([], C.c_double),
+    "qlt_set_row_panels": ([C.c_int32], C.c_int32),
+    "qlt_get_row_panels": ([], C.c_int32),
     "qlt_kernel_probe_set": ([C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_kernel_probe_last": ([C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int32),
     "qlt_graph_begin": ([C.c_void_p], C.c_int32),

@@ -57,7 +57,7 @@ qlt_status qlt_contract_execute(const qlt_plan *plan, int32_t dtype, int32_t mod
   QLT_CHECK_ARG(alpha != nullptr && beta != nullptr, "qlt_contract_execute: null alpha/beta");
   QLT_CHECK_ARG(plan->device == current_device(),
                 "qlt_contract_execute: plan was built on device " + std::to_string(plan->device));
-  if (plan->n_tiles == 0) return QLT_OK;
+  if (plan->n_tiles == 0 && plan->n_ptiles == 0) return QLT_OK;
   QLT_CHECK_ARG(a != nullptr && b != nullptr && c != nullptr, "qlt_contract_execute: null data buffer");
   hipStream_t st = as_stream(stream);
   if (mode == QLT_MODE_NATIVE || dtype_is_single(dtype)) return launch_native(plan, dtype, a, b, c, alpha, beta, st);

@@ -694,6 +694,136 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restric
   rows_epilogue<CPLX, NMAX, RPT, E>(seg, it.tm * (256 * RPT), tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
+// Row panels (PanelTile): one 1024-row tile of a panel per workgroup, 4 rows
+// per thread.  While each thread loads the A' elements of its rows in every
+// column of the panel, the panel's terms (their B' elements, one thread
+// each) and output columns are staged in LDS; then each output column is
+// summed from registers in the row kernel's order and stored with its alpha
+// / beta epilogue.  Replaces, for these blocks, the per-task loop of
+// global_operations.h:919-978 (one BLAS call per task).
+template <bool CPLX, typename E = double>
+__global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict__ tiles,
+                                                    const PanelOut *__restrict__ outs,
+                                                    const PanelTerm *__restrict__ terms, const Seg *__restrict__ segs,
+                                                    const E *__restrict__ A, const E *__restrict__ B,
+                                                    E *__restrict__ C, double alpha_r, double alpha_i,
+                                                    double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  constexpr int RPT = kPanelRpt;
+  __shared__ double s_br[kPanelMaxTerms], s_bi[kPanelMaxTerms];
+  __shared__ int64_t s_ocoff[kPanelMaxOuts];
+  __shared__ int32_t s_on[kPanelMaxOuts], s_ot0[kPanelMaxOuts], s_omask[kPanelMaxOuts];
+  static_assert(kPanelMaxTerms <= 128 && kPanelMaxOuts <= 128, "one staging thread per term and per output");
+
+  const PanelTile &pt = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int n_cols = pt.n_cols, n_outs = pt.n_outs, n_terms = pt.n_terms;
+
+  // row offsets in every column: the row's digits over rdim (axes past the
+  // saved ones have extent 1 and stride 0), dotted with each column's
+  // strides.  Rows past the block read its first row (in bounds); their
+  // sums are never stored, and the columns past n_cols are never read.
+  int32_t roff[kPanelCols][RPT];
+  bool valid[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int row = pt.r0 + tid + 256 * q;
+    valid[q] = row < pt.m;
+    int32_t x = valid[q] ? row : 0;
+#pragma unroll
+    for (int c = 0; c < kPanelCols; ++c) roff[c][q] = 0;
+#pragma unroll
+    for (int l = kPanelRowAxes - 1; l >= 0; --l) {
+      const int32_t qd = static_cast<int32_t>((static_cast<uint64_t>(x) * pt.rmag[l]) >> pt.rsh[l]);
+      const int32_t d = x - qd * pt.rdim[l];
+      x = qd;
+#pragma unroll
+      for (int c = 0; c < kPanelCols; ++c) roff[c][q] += d * pt.cstr[c][l];
+    }
+  }
+  // every A' load is issued before anything waits on memory
+  double av[kPanelCols][RPT], av_i[kPanelCols][RPT];
+#pragma unroll
+  for (int c = 0; c < kPanelCols; ++c)
+    if (c < n_cols) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const double2 v = ld_elem<E, CPLX>(A + (pt.cbase[c] + roff[c][q]) * P);
+        av[c][q] = v.x;
+        av_i[c][q] = v.y;
+      }
+    }
+  // stage the terms (B' elements) and the output columns
+  if (tid < n_terms) {
+    const PanelTerm tm = terms[pt.term0 + tid];
+    const double2 v = ld_elem<E, CPLX>(B + tm.b * P);
+    s_br[tid] = v.x;
+    s_bi[tid] = v.y;
+  } else if (tid >= 128 && tid < 128 + n_outs) {
+    const int o = tid - 128;
+    const PanelOut po = outs[pt.out0 + o];
+    const Seg &sg = segs[po.seg];
+    s_ocoff[o] = sg.c_off + po.j;
+    s_on[o] = sg.n;
+    s_ot0[o] = po.term0;
+    s_omask[o] = po.mask;
+  }
+  __syncthreads();
+
+  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  for (int o = 0; o < n_outs; ++o) {
+    const int64_t coff = s_ocoff[o];
+    const int32_t n = __builtin_amdgcn_readfirstlane(s_on[o]);
+    const int32_t mask = __builtin_amdgcn_readfirstlane(s_omask[o]);
+    int32_t t = __builtin_amdgcn_readfirstlane(s_ot0[o]);
+    double y[RPT], y_i[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      y[q] = 0.0;
+      y_i[q] = 0.0;
+    }
+    // the output's terms in ascending column order, which is their (task, k)
+    // order: the column loop is static, so no register array is indexed
+#pragma unroll
+    for (int c = 0; c < kPanelCols; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      const double b = s_br[t], b_im = s_bi[t];
+      ++t;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        if (CPLX) {
+          y[q] = fma(av[c][q], b, y[q]);
+          y[q] = fma(-av_i[c][q], b_im, y[q]);
+          y_i[q] = fma(av[c][q], b_im, y_i[q]);
+          y_i[q] = fma(av_i[c][q], b, y_i[q]);
+        } else {
+          y[q] = fma(av[c][q], b, y[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      if (!valid[q]) continue;
+      const int64_t e = coff + static_cast<int64_t>(pt.r0 + tid + 256 * q) * n;
+      if (CPLX) {
+        double yr = alpha_r * y[q] - alpha_i * y_i[q];
+        double yi = alpha_r * y_i[q] + alpha_i * y[q];
+        E *cp = C + 2 * e;
+        if (use_beta) {
+          const double2 old = ld_elem<E, true>(cp);
+          yr += beta_r * old.x - beta_i * old.y;
+          yi += beta_r * old.y + beta_i * old.x;
+        }
+        st_elem<E, true>(cp, yr, yi);
+      } else {
+        double v = alpha_r * y[q];
+        if (use_beta) v += beta_r * static_cast<double>(C[e]);
+        st_elem<E, false>(C + e, v, 0.0);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 namespace qlt {
@@ -707,15 +837,21 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
   const bool cplx = dtype_is_complex(dtype);
   // classes with work run on the class streams when there are two or more
   int n_cls = 0;
-  for (int32_t cls = mfma_class ? 0 : kNumMfmaClasses; cls < kNumTileClasses; ++cls) n_cls += plan->cls_tiles[cls] > 0;
+  for (int32_t cls = mfma_class ? 0 : kNumMfmaClasses; cls < kNumTileClasses; ++cls)
+    n_cls += plan->cls_tiles[cls] > 0 || (cls == kTileRows4 && plan->n_ptiles > 0);
   std::unique_ptr<ClassStreams> own;
   ClassStreams *cs = shared;
   if (cs == nullptr && n_cls >= 2) {
     own = std::make_unique<ClassStreams>(st);
     cs = own.get();
   }
-  auto stream_of = [&](int32_t cls) { return cs != nullptr ? cs->get(cls) : st; };
+  auto stream_of = [&](int32_t cls) {
+    if (cls == kNumTileClasses) cls = kTileRows4;   // the row panels
+    return cs != nullptr ? cs->get(cls) : st;
+  };
   // the class with the most algorithmic work is the one a kernel probe times
+  // (kPanelId: the row panels, on the stream of the n <= 4 row class)
+  constexpr int32_t kPanelId = kNumTileClasses;
   int32_t dom = -1;
   if (probe && probe_armed(st)) {
     double best = 0.0;
@@ -724,6 +860,7 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
         best = plan->cls_mkn[cls];
         dom = cls;
       }
+    if (plan->n_ptiles > 0 && plan->panel_mkn > best) dom = kPanelId;
   }
   auto probe_in = [&](int32_t cls) {
     if (cls == dom) probe_begin(stream_of(cls));
@@ -731,13 +868,16 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
   auto probe_out = [&](int32_t cls) {
     if (cls != dom) return;
     const hipStream_t ps = stream_of(cls);
-    static const char *const kNames[kNumTileClasses] = {"k_grouped_gemm_f64<64x64>", "k_grouped_gemm_f64<32x64>",
-                                                        "k_grouped_gemm_f64<64x32>", "k_grouped_gemm_f64<32x32>",
-                                                        "k_rows_f64<n<=4>", "k_rows_f64<n<=16>"};
+    static const char *const kNames[kNumTileClasses + 1] = {
+        "k_grouped_gemm_f64<64x64>", "k_grouped_gemm_f64<32x64>", "k_grouped_gemm_f64<64x32>",
+        "k_grouped_gemm_f64<32x32>", "k_rows_f64<n<=4>",          "k_rows_f64<n<=16>",
+        "k_panel_rows"};
     char name[64];
     static const char *const kTypes[4] = {"f64", "c128", "f32", "c64"};
     std::snprintf(name, sizeof(name), "%s %s", kNames[cls], kTypes[dtype & 3]);
-    probe_end(ps, name, (cplx ? 8.0 : 2.0) * plan->cls_mkn[cls], static_cast<double>(dtype_bytes(dtype)) * plan->cls_elems[cls]);
+    const double mkn = cls == kPanelId ? plan->panel_mkn : plan->cls_mkn[cls];
+    const double elems = cls == kPanelId ? plan->panel_elems : plan->cls_elems[cls];
+    probe_end(ps, name, (cplx ? 8.0 : 2.0) * mkn, static_cast<double>(dtype_bytes(dtype)) * elems);
   };
   const double ar = alpha[0], ai = cplx ? alpha[1] : 0.0;
   const double br = beta[0], bi = cplx ? beta[1] : 0.0;
@@ -808,6 +948,14 @@ qlt_status launch_native_classes(const qlt_plan *plan, int32_t dtype, const void
       };
       if (cls == kTileRows4) rows(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
       else rows(std::integral_constant<int, 16>{}, std::integral_constant<int, 1>{});
+    }
+    // row panels, after the n <= 4 row class on its stream
+    if (plan->n_ptiles > 0) {
+      const hipStream_t ls = stream_of(kPanelId);
+      probe_in(kPanelId);
+      hipLaunchKernelGGL((k_panel_rows<CX, E>), dim3(static_cast<unsigned>(plan->n_ptiles)), block, 0, ls,
+                         plan->d_ptiles, plan->d_pouts, plan->d_pterms, plan->d_segs, A, B, Cp, ar, ai, br, bi);
+      probe_out(kPanelId);
     }
   };
   switch (dtype) {

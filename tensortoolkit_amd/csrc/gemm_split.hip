@@ -1652,7 +1652,8 @@ qlt_status launch_split(const qlt_plan *plan, int32_t dtype, int32_t mode, const
   const dim3 blk(256);
   // the row classes (if any) go on the class streams, forked before the pack
   std::unique_ptr<ClassStreams> rows_cs;
-  if (plan->cls_tiles[kTileRows4] > 0 || plan->cls_tiles[kTileRows16] > 0) rows_cs = std::make_unique<ClassStreams>(st);
+  if (plan->cls_tiles[kTileRows4] > 0 || plan->cls_tiles[kTileRows16] > 0 || plan->n_ptiles > 0)
+    rows_cs = std::make_unique<ClassStreams>(st);
   // a kernel probe times the split GEMM unless the row classes carry more work
   double mfma_mkn = 0.0, rows_mkn = 0.0;
   for (int32_t cls = 0; cls < kNumTileClasses; ++cls) (is_mfma_class(cls) ? mfma_mkn : rows_mkn) += plan->cls_mkn[cls];

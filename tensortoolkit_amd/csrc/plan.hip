@@ -19,6 +19,7 @@
 // output block, and the output tiles ordered longest first.
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
 #include <cstring>
 #include <numeric>
@@ -502,6 +503,183 @@ qlt_status mark_plain_segments(qlt_plan *plan) {
   return QLT_OK;
 }
 
+// Row panels on (1) or off (0) for plans built from now on (qlt_set_row_panels).
+std::atomic<int> g_row_panels{1};
+
+// Host form of split_offset (plain division).
+int64_t host_split_offset(int64_t x, const int32_t *dims, const int32_t *strides, int n) {
+  int64_t off = 0;
+  for (int j = n - 1; j >= 0; --j) {
+    const int64_t q = x / dims[j];
+    off += (x - q * dims[j]) * strides[j];
+    x = q;
+  }
+  return off;
+}
+
+// Row panels of the n <= 4 row class (PanelTile).  The output blocks are in
+// ascending block order with A's saved axes leading, so the blocks sharing
+// their coordinates on those axes are consecutive.  Each run of such blocks
+// whose n <= 4 members together touch at most kPanelCols A' columns, with at
+// most kPanelMaxOuts output columns and kPanelMaxTerms terms, becomes one
+// panel; in_panel[o] marks its output blocks, which leave the row class.
+qlt_status build_panels(qlt_plan *plan, std::vector<char> &in_panel) {
+  const int64_t n_c = plan->n_segs;
+  in_panel.assign(static_cast<size_t>(n_c), 0);
+  const int nr = plan->layout.a_nr;
+  if (n_c == 0 || nr < 1 || nr > kPanelRowAxes || plan->c_rank < nr || plan->c_coords.empty()) return QLT_OK;
+  const auto &segs = plan->h_segs;
+  const auto &ex = plan->h_exec;
+  std::vector<BlkGather> ga(static_cast<size_t>(plan->n_a_blks)), gb(static_cast<size_t>(plan->n_b_blks));
+  QLT_HIP(hipMemcpy(ga.data(), plan->d_ga, sizeof(BlkGather) * ga.size(), hipMemcpyDeviceToHost));
+  QLT_HIP(hipMemcpy(gb.data(), plan->d_gb, sizeof(BlkGather) * gb.size(), hipMemcpyDeviceToHost));
+  const int32_t cr = plan->c_rank;
+  const GatherLayout &L = plan->layout;
+  std::vector<PanelTile> tiles;
+  std::vector<PanelOut> outs;
+  std::vector<PanelTerm> terms;
+  int64_t o0 = 0;
+  while (o0 < n_c) {
+    // the run of output blocks with the coordinates of block o0 on A's saved axes
+    int64_t o1 = o0 + 1;
+    auto same_rows = [&](int64_t a, int64_t b) {
+      for (int j = 0; j < nr; ++j)
+        if (plan->c_coords[a * cr + j] != plan->c_coords[b * cr + j]) return false;
+      return true;
+    };
+    while (o1 < n_c && same_rows(o0, o1)) ++o1;
+    // its n <= 4 blocks, their A' columns (block, contracted index) in
+    // ascending order, and per output column the terms in (task, k) order,
+    // which must be ascending in the column
+    std::vector<std::pair<int32_t, int32_t>> cols;
+    std::vector<int64_t> members;
+    std::vector<PanelOut> p_outs;
+    std::vector<PanelTerm> p_terms;
+    bool ok = true;
+    int32_t m = -1;
+    for (int64_t o = o0; o < o1 && ok; ++o) {
+      const Seg &sg = segs[static_cast<size_t>(o)];
+      if (sg.cls != kTileRows4) continue;
+      if (m >= 0 && sg.m != m) ok = false;
+      m = sg.m;
+      members.push_back(o);
+      for (int32_t t = sg.t0; t < sg.t1; ++t) {
+        const ExecTask &e = ex[static_cast<size_t>(t)];
+        for (int32_t p = 0; p < e.k; ++p) cols.emplace_back(e.a_blk, p);
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    ok = ok && !members.empty() && m > 0 && static_cast<int64_t>(cols.size()) <= kPanelCols;
+    for (size_t mi = 0; mi < members.size() && ok; ++mi) {
+      const Seg &sg = segs[static_cast<size_t>(members[mi])];
+      for (int32_t j = 0; j < sg.n && ok; ++j) {
+        PanelOut po{static_cast<int32_t>(members[mi]), j, static_cast<int32_t>(p_terms.size()), 0};
+        int32_t last = -1;
+        for (int32_t t = sg.t0; t < sg.t1 && ok; ++t) {
+          const ExecTask &e = ex[static_cast<size_t>(t)];
+          const BlkGather &gB = gb[static_cast<size_t>(e.b_blk)];
+          const int64_t bcol = host_split_offset(j, gB.cdim, gB.cstr, L.b_nc);
+          for (int32_t p = 0; p < e.k && ok; ++p) {
+            const int32_t c = static_cast<int32_t>(
+                std::lower_bound(cols.begin(), cols.end(), std::make_pair(e.a_blk, p)) - cols.begin());
+            if (c <= last) {
+              ok = false;
+              break;
+            }
+            last = c;
+            po.mask |= 1 << c;
+            const int64_t brow = host_split_offset(p, gB.rdim, gB.rstr, L.b_nr);
+            p_terms.push_back(PanelTerm{e.b_off + brow + bcol, c, 0});
+          }
+        }
+        p_outs.push_back(po);
+      }
+    }
+    ok = ok && static_cast<int64_t>(p_outs.size()) <= kPanelMaxOuts &&
+         static_cast<int64_t>(p_terms.size()) <= kPanelMaxTerms;
+    if (ok) {
+      PanelTile pt{};
+      const BlkGather &g0 = ga[static_cast<size_t>(cols[0].first)];
+      for (int l = 0; l < kPanelRowAxes; ++l) {
+        pt.rdim[l] = l < nr ? g0.rdim[l] : 1;
+        pt.rmag[l] = g0.rmag[l];
+        pt.rsh[l] = g0.rsh[l];
+      }
+      for (int c = 0; c < kPanelCols; ++c) {
+        pt.cbase[c] = 0;
+        for (int l = 0; l < kPanelRowAxes; ++l) pt.cstr[c][l] = 0;
+      }
+      for (size_t c = 0; c < cols.size(); ++c) {
+        const BlkGather &g = ga[static_cast<size_t>(cols[c].first)];
+        // the A block's element offset comes with any task that reads it
+        int64_t a_off = -1;
+        for (int64_t o : members) {
+          const Seg &sg = segs[static_cast<size_t>(o)];
+          for (int32_t t = sg.t0; t < sg.t1 && a_off < 0; ++t)
+            if (ex[static_cast<size_t>(t)].a_blk == cols[c].first) a_off = ex[static_cast<size_t>(t)].a_off;
+        }
+        pt.cbase[c] = a_off + host_split_offset(cols[c].second, g.cdim, g.cstr, L.a_nc);
+        for (int l = 0; l < nr; ++l) pt.cstr[c][l] = g.rstr[l];
+      }
+      pt.m = m;
+      pt.n_cols = static_cast<int32_t>(cols.size());
+      pt.n_outs = static_cast<int32_t>(p_outs.size());
+      pt.n_terms = static_cast<int32_t>(p_terms.size());
+      pt.out0 = static_cast<int32_t>(outs.size());
+      pt.term0 = static_cast<int32_t>(terms.size());
+      for (int32_t r0 = 0; r0 < m; r0 += kPanelRows) {
+        pt.r0 = r0;
+        tiles.push_back(pt);
+      }
+      outs.insert(outs.end(), p_outs.begin(), p_outs.end());
+      terms.insert(terms.end(), p_terms.begin(), p_terms.end());
+      for (int64_t o : members) {
+        in_panel[static_cast<size_t>(o)] = 1;
+        const Seg &sg = segs[static_cast<size_t>(o)];
+        for (int32_t t = sg.t0; t < sg.t1; ++t)
+          plan->panel_mkn += static_cast<double>(sg.m) * static_cast<double>(ex[static_cast<size_t>(t)].k) * sg.n;
+        plan->panel_elems += static_cast<double>(sg.m) * sg.n;
+      }
+      plan->panel_elems += static_cast<double>(m) * static_cast<double>(cols.size());
+    }
+    o0 = o1;
+  }
+  QLT_CHECK_ARG(static_cast<int64_t>(terms.size()) < (int64_t(1) << 31) &&
+                    static_cast<int64_t>(tiles.size()) < (int64_t(1) << 31),
+                "qlt_plan_build: too many row panels");
+  plan->n_ptiles = static_cast<int64_t>(tiles.size());
+  if (tiles.empty()) return QLT_OK;
+  QLT_HIP(hipMalloc(&plan->d_ptiles, sizeof(PanelTile) * tiles.size()));
+  QLT_HIP(hipMalloc(&plan->d_pouts, sizeof(PanelOut) * outs.size()));
+  QLT_HIP(hipMalloc(&plan->d_pterms, sizeof(PanelTerm) * terms.size()));
+  QLT_HIP(hipMemcpy(plan->d_ptiles, tiles.data(), sizeof(PanelTile) * tiles.size(), hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(plan->d_pouts, outs.data(), sizeof(PanelOut) * outs.size(), hipMemcpyHostToDevice));
+  QLT_HIP(hipMemcpy(plan->d_pterms, terms.data(), sizeof(PanelTerm) * terms.size(), hipMemcpyHostToDevice));
+  // what the row class keeps
+  plan->cls_mkn[kTileRows4] = 0.0;
+  plan->cls_elems[kTileRows4] = 0.0;
+  std::vector<char> seen_a(static_cast<size_t>(plan->n_a_blks), 0), seen_b(static_cast<size_t>(plan->n_b_blks), 0);
+  for (int64_t o = 0; o < n_c; ++o) {
+    const Seg &sg = segs[static_cast<size_t>(o)];
+    if (sg.cls != kTileRows4 || in_panel[static_cast<size_t>(o)]) continue;
+    plan->cls_mkn[kTileRows4] += static_cast<double>(sg.m) * sg.n * static_cast<double>(sg.k_total);
+    plan->cls_elems[kTileRows4] += static_cast<double>(sg.m) * sg.n;
+    for (int32_t t = sg.t0; t < sg.t1; ++t) {
+      const ExecTask &e = ex[static_cast<size_t>(t)];
+      if (!seen_a[e.a_blk]) {
+        seen_a[e.a_blk] = 1;
+        plan->cls_elems[kTileRows4] += static_cast<double>(sg.m) * e.k;
+      }
+      if (!seen_b[e.b_blk]) {
+        seen_b[e.b_blk] = 1;
+        plan->cls_elems[kTileRows4] += static_cast<double>(e.k) * sg.n;
+      }
+    }
+  }
+  return QLT_OK;
+}
+
 // k-slicing of long tiles (MFMA and row classes alike).  Per class, a step budget Q = max(kMinSliceChunks,
 // ceil(total steps / T)) is set, T = kSliceTargetWGs unless tuned (qlt_tuning); a tile of w > Q steps is cut into
 // ceil(w / Q) (at most kMaxSlices) slices of nearly equal length along its
@@ -525,6 +703,8 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
   std::vector<int64_t> seg_steps(segs.size(), 0);
   for (size_t o = 0; o < segs.size(); ++o)
     for (int32_t t = segs[o].t0; t < segs[o].t1; ++t) seg_steps[o] += steps_of(ex[static_cast<size_t>(t)].k);
+  std::vector<char> in_panel;
+  if (plan->panels_enabled) QLT_TRY(build_panels(plan, in_panel));
   std::vector<Tile> out;
   std::vector<ReduceItem> red;
   out.reserve(static_cast<size_t>(n));
@@ -535,7 +715,8 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
     const int64_t slice_doubles = is_mfma_class(cls) ? int64_t(mfma_tile_m(cls)) * mfma_tile_n(cls) * 2
                                                      : int64_t(rows_per_tile(cls)) * (cls == kTileRows4 ? 4 : 16) * 2;
     int64_t total = 0;
-    for (int64_t i = base; i < base + cnt; ++i) total += seg_steps[static_cast<size_t>(h[i].seg)];
+    for (int64_t i = base; i < base + cnt; ++i)
+      if (in_panel.empty() || !in_panel[static_cast<size_t>(h[i].seg)]) total += seg_steps[static_cast<size_t>(h[i].seg)];
     // tuned workgroup target of the class (qlt_tuning; -1: unsliced)
     const bool unsliced = plan->slice_target[cls] < 0;
     const int64_t target = plan->slice_target[cls] > 0 ? plan->slice_target[cls] : kSliceTargetWGs;
@@ -544,6 +725,7 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
     int64_t parts = 0, n_red = 0;
     for (int64_t i = base; i < base + cnt; ++i) {
       Tile t = h[static_cast<size_t>(i)];
+      if (!in_panel.empty() && in_panel[static_cast<size_t>(t.seg)]) continue;   // runs as a row panel
       const Seg &s = segs[static_cast<size_t>(t.seg)];
       const int64_t w = seg_steps[static_cast<size_t>(t.seg)];
       const int64_t ns = unsliced ? 1 : std::min(kMaxSlices, (w + q - 1) / q);
@@ -628,6 +810,12 @@ void free_plan_device(qlt_plan *p) {
   if (p->d_work) (void)hipFree(p->d_work);
   if (p->d_red) (void)hipFree(p->d_red);
   if (p->d_part) (void)hipFree(p->d_part);
+  if (p->d_ptiles) (void)hipFree(p->d_ptiles);
+  if (p->d_pouts) (void)hipFree(p->d_pouts);
+  if (p->d_pterms) (void)hipFree(p->d_pterms);
+  p->d_ptiles = nullptr;
+  p->d_pouts = nullptr;
+  p->d_pterms = nullptr;
   p->d_red = nullptr;
   p->d_part = nullptr;
   p->d_exec = nullptr;
@@ -1082,10 +1270,19 @@ qlt_status qlt_plan_build(const qlt_bsdt_desc *a, const qlt_bsdt_desc *b, int32_
     plan->sum_mkn = sum;
   }
   QLT_TRY(mark_plain_segments(plan.get()));
+  plan->panels_enabled = g_row_panels.load() != 0;
   QLT_TRY(slice_long_tiles(plan.get()));
   *plan_out = plan.release();
   return QLT_OK;
 }
+
+qlt_status qlt_set_row_panels(int32_t enable) {
+  QLT_CHECK_ARG(enable == 0 || enable == 1, "qlt_set_row_panels: enable must be 0 or 1");
+  g_row_panels.store(enable);
+  return QLT_OK;
+}
+
+int32_t qlt_get_row_panels(void) { return g_row_panels.load(); }
 
 qlt_status qlt_plan_destroy(qlt_plan *plan) {
   delete plan;   // the destructor releases the device work list
@@ -1103,6 +1300,7 @@ qlt_status qlt_plan_get_info(const qlt_plan *plan, qlt_plan_info *info) {
   info->n_tiles = plan->n_tiles;
   info->a_unique_elems = plan->a_unique_elems;
   info->b_unique_elems = plan->b_unique_elems;
+  info->n_panel_tiles = plan->n_ptiles;
   return QLT_OK;
 }
 

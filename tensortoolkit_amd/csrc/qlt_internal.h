@@ -202,6 +202,59 @@ struct Tile {
   int32_t part;
 };
 
+// Row panels.  The output blocks of the n <= 4 row class that share their
+// coordinates on A's saved axes read the same rows of A' (each from its own
+// A blocks, with the same row digits and per-block strides).  When those
+// blocks together touch at most kPanelCols distinct A' columns (block,
+// contracted index), they run as one panel: a workgroup loads each A element
+// of its rows once into registers and forms every output column of every
+// block of the panel from them, where the row kernel would re-read an A
+// block once per output block that uses it.  An output column is the sum of
+// its terms (A' column, B' element) in (task, k) order, the order of the row
+// kernel, so both give bitwise the same values.
+constexpr int kPanelCols = 4;        // distinct A' columns per panel
+constexpr int kPanelRowAxes = 4;     // A saved axes
+constexpr int kPanelMaxOuts = 16;    // output columns per panel
+constexpr int kPanelMaxTerms = 64;   // terms per panel
+constexpr int kPanelRpt = 4;         // rows per thread (256 threads)
+constexpr int kPanelRows = 256 * kPanelRpt;
+
+// One 1024-row tile of a panel.  Column c's element of row i sits at
+// A + cbase[c] + sum_l digit_l(i) * cstr[c][l], the digits of i taken over
+// rdim (last fastest) with the multiply-shift reciprocals rmag / rsh.
+struct PanelTile {
+  uint64_t rmag[kPanelRowAxes];
+  int32_t rdim[kPanelRowAxes];
+  int32_t rsh[kPanelRowAxes];
+  int64_t cbase[kPanelCols];
+  int32_t cstr[kPanelCols][kPanelRowAxes];
+  int32_t m;        // rows of the panel's blocks
+  int32_t r0;       // first row of this tile
+  int32_t n_cols;   // A' columns
+  int32_t n_outs;   // output columns (PanelOut records from out0)
+  int32_t n_terms;  // terms (PanelTerm records from term0)
+  int32_t out0;
+  int32_t term0;
+  int32_t pad;
+};
+// Output column j of output block `seg`: one term per set bit c of `mask`
+// (A' column c of the panel), in ascending c, stored from term0 on (indices
+// relative to the panel's first term).  The panel's columns are numbered in
+// ascending (A block, contracted index) order, and an output block's tasks
+// run in ascending A block order, so ascending c is the (task, k) order.
+struct PanelOut {
+  int32_t seg;
+  int32_t j;
+  int32_t term0;
+  int32_t mask;
+};
+// One term: an A' column of the panel times the B element at `b`.
+struct PanelTerm {
+  int64_t b;
+  int32_t col;
+  int32_t pad;
+};
+
 // Output tile whose k-slices are summed by the reduction kernel: partial
 // buffers part0 .. part0 + n_parts - 1, in slice order.
 struct ReduceItem {
@@ -305,6 +358,14 @@ struct qlt_plan {
   double *d_part = nullptr;
   size_t part_doubles = 0;
   int64_t cls_part_off[qlt::kNumTileClasses] = {0, 0, 0, 0, 0, 0};
+  // ---- row panels (output blocks of the n <= 4 row class sharing A rows) ----
+  bool panels_enabled = true;           // qlt_set_row_panels in effect at build time
+  int64_t n_ptiles = 0;
+  qlt::PanelTile *d_ptiles = nullptr;
+  qlt::PanelOut *d_pouts = nullptr;
+  qlt::PanelTerm *d_pterms = nullptr;
+  double panel_mkn = 0.0;               // sum of m*k*n over the panels' tasks
+  double panel_elems = 0.0;             // distinct A elements + C elements of the panels
   // ---- split-precision workspace ----
   void *d_work = nullptr;
   size_t work_bytes = 0;
@@ -322,7 +383,8 @@ struct qlt_plan {
   ~qlt_plan() {
     for (void *p : {static_cast<void *>(d_exec), static_cast<void *>(d_segs), static_cast<void *>(d_tiles),
                     static_cast<void *>(d_ga), static_cast<void *>(d_gb), d_work, static_cast<void *>(d_red),
-                    static_cast<void *>(d_part)})
+                    static_cast<void *>(d_part), static_cast<void *>(d_ptiles), static_cast<void *>(d_pouts),
+                    static_cast<void *>(d_pterms)})
       if (p != nullptr) (void)hipFree(p);
   }
 };
