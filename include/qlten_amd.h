@@ -113,7 +113,7 @@ typedef struct {
   int64_t n_tiles;       /* output tiles of the grouped kernels (row panels aside)  */
   int64_t a_unique_elems;/* elements of A blocks touched by at least one task      */
   int64_t b_unique_elems;/* elements of B blocks touched by at least one task      */
-  int64_t n_panel_tiles; /* 1024-row tiles of the row panels (qlt_set_row_panels)  */
+  int64_t n_panel_tiles; /* 512-row tiles of the row panels (qlt_set_row_panels)   */
 } qlt_plan_info;
 
 /* ---- runtime ---------------------------------------------------------- */

@@ -109,13 +109,14 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   // blocks always read along memory order
   bool akf = L.a_kfast != 0, bnf = L.b_nfast != 0;
 
-  v4d acc[BM][BN], acc_i[BM][BN];
+  v4d acc[BM][BN], acc_i[BM][BN], acc_s[BM][BN];
 #pragma unroll
   for (int a = 0; a < BM; ++a)
 #pragma unroll
     for (int b = 0; b < BN; ++b) {
       acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+      acc_s[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
   auto a_r = [&](int q) { return akf ? (tid >> 4) + 16 * (q % QA0) : tid % TMv; };
@@ -208,15 +209,21 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
         for (int mt = 0; mt < BM; ++mt) ai[mt] = As[buf][P - 1][ks + lk][wm + mt * 16 + lr];
 #pragma unroll
         for (int nt = 0; nt < BN; ++nt) bi[nt] = Bs[buf][P - 1][ks + lk][wn + nt * 16 + lr];
-        // re += ar*br - ai*bi ; im += ar*bi + ai*br
+        // three real products per complex one (Gauss): acc = sum ar*br (above),
+        // acc_i = sum ai*bi, acc_s = sum (ar + ai)*(br + bi); after the last
+        // step re = acc - acc_i and im = acc_s - acc - acc_i
+        double as[BM], bs[BN];
+#pragma unroll
+        for (int mt = 0; mt < BM; ++mt) as[mt] = a[mt] + ai[mt];
+#pragma unroll
+        for (int nt = 0; nt < BN; ++nt) bs[nt] = b[nt] + bi[nt];
 #pragma unroll
         for (int mt = 0; mt < BM; ++mt)
 #pragma unroll
           for (int nt = 0; nt < BN; ++nt) {
             if (!(mt_ok[mt] && nt_ok[nt])) continue;
-            acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bi[nt], acc_i[mt][nt], 0, 0, 0);
-            acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[mt], bi[nt], acc[mt][nt], 0, 0, 0);
-            acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[mt], b[nt], acc_i[mt][nt], 0, 0, 0);
+            acc_i[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[mt], bi[nt], acc_i[mt][nt], 0, 0, 0);
+            acc_s[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[mt], bs[nt], acc_s[mt][nt], 0, 0, 0);
           }
       }
     }
@@ -370,6 +377,18 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       sp = sp2;
       buf ^= 1;
     }
+  }
+
+  // ---- complex: the real and imaginary parts from the three product sums ----
+  if (CPLX) {
+#pragma unroll
+    for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt) {
+        const v4d p1 = acc[mt][nt], p2 = acc_i[mt][nt];
+        acc[mt][nt] = p1 - p2;
+        acc_i[mt][nt] = acc_s[mt][nt] - p1 - p2;
+      }
   }
 
   // ---- a k-slice stores its unscaled partial sum, in the thread's own order ----
@@ -694,7 +713,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restric
   rows_epilogue<CPLX, NMAX, RPT, E>(seg, it.tm * (256 * RPT), tid, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
-// Row panels (PanelTile): one 1024-row tile of a panel per workgroup, 4 rows
+// Row panels (PanelTile): one 512-row tile of a panel per workgroup, 2 rows
 // per thread.  While each thread loads the A' elements of its rows in every
 // column of the panel, the panel's terms (their B' elements, one thread
 // each) and output columns are staged in LDS; then each output column is

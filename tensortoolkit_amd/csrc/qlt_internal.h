@@ -216,10 +216,10 @@ constexpr int kPanelCols = 4;        // distinct A' columns per panel
 constexpr int kPanelRowAxes = 4;     // A saved axes
 constexpr int kPanelMaxOuts = 16;    // output columns per panel
 constexpr int kPanelMaxTerms = 64;   // terms per panel
-constexpr int kPanelRpt = 4;         // rows per thread (256 threads)
+constexpr int kPanelRpt = 2;         // rows per thread (256 threads)
 constexpr int kPanelRows = 256 * kPanelRpt;
 
-// One 1024-row tile of a panel.  Column c's element of row i sits at
+// One 512-row tile of a panel.  Column c's element of row i sits at
 // A + cbase[c] + sum_l digit_l(i) * cstr[c][l], the digits of i taken over
 // rdim (last fastest) with the multiply-shift reciprocals rmag / rsh.
 struct PanelTile {
