@@ -385,7 +385,7 @@ def run_e2e(args) -> int:
     print(f"[e2e] gpu lanczos, one hipGraph per iteration replayed: {lz.seconds:.3f}s "
           f"({lz.matvec_seconds:.3f}s replaying), eigenvalue {lz.eigenvalue!r}", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    res = DEC.svd(mv.psi, 2, (0, 0), cutoff, 1, dmax, t_dev=lz.ground_state, stream=stream)
+    res = DEC.svd(mv.psi, 2, (0, 0), cutoff, 1, dmax, t_dev=lz.ground_state, stream=stream, to_host=False)
     svd_s = time.perf_counter() - t1
     print(f"[e2e] gpu svd: {svd_s:.3f}s, D {res.D}, trunc err {res.trunc_err!r}, sweeps {res.sweeps}",
           file=sys.stderr, flush=True)

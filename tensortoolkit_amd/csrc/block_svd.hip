@@ -10,12 +10,17 @@
 // SVD.  The working matrix X is A (rows >= cols) or A^H (rows < cols), stored
 // column-major so that columns are contiguous; V starts as the identity.  The
 // columns are split into blocks of kB; every round of a sweep pairs the
-// blocks by the circle method and one workgroup per block pair
-//   1. forms the Gram matrix G = P^H P of the panel P = [X_I X_J],
-//   2. skips the pair when max |G_pq| / sqrt(G_pp G_qq) < tol,
-//   3. diagonalises G in LDS by cyclic two-sided Jacobi, accumulating the
-//      unitary Q (G <- J^H G J, Q <- Q J per rotation),
-//   4. updates P <- P Q and the matching columns of V <- V Q.
+// blocks by the circle method, and for every block pair
+//   1. one workgroup per kRG-row group forms that group's part of the Gram
+//      matrix G = P^H P of the panel P = [X_I X_J],
+//   2. one workgroup sums the parts in group order, skips the pair when
+//      max |G_pq| / sqrt(G_pp G_qq) < tol, and otherwise diagonalises G in
+//      LDS by cyclic two-sided Jacobi (at most kInnerSweeps sweeps),
+//      accumulating the unitary Q (G <- J^H G J, Q <- Q J per rotation),
+//   3. one workgroup per kRG-row group updates P <- P Q and the matching
+//      columns of V <- V Q.
+// Spreading the rows over workgroups keeps the chip busy on the few large
+// block matrices, whose pairs set the length of a round.
 // Before every sweep the columns are ordered by decreasing norm (the block
 // order follows it), which keeps graded spectra to a few sweeps.  A matrix
 // whose sweep rotates nothing has converged; then sigma_j = |X_j|,
@@ -90,8 +95,11 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kB = 16;          // columns per Jacobi block
 constexpr int kW = 2 * kB;      // panel width
 constexpr int kRC = 16;         // rows per staged panel chunk
+constexpr int kRG = 64;         // rows per Gram / update workgroup
 constexpr int kMaxSweeps = 40;
-constexpr int kInnerSweeps = 30;
+// Sweeps of the pair's Jacobi eigen-solve per visit: the outer sweeps repeat
+// the pair until its columns are orthogonal, so the solve need not finish.
+constexpr int kInnerSweeps = 1;
 constexpr int kMaxSortCols = 4096;   // LDS bound of the per-matrix column sort
 
 struct SvdMat {
@@ -103,7 +111,10 @@ struct SvdMat {
   int64_t a_off, u_off, s_off, vt_off;
   int32_t nb, P;      // column blocks; players of the circle method (even)
   int32_t tr;         // X = A^H
+  int32_t ng;         // row groups of X (kRG rows each)
+  int32_t pair0;      // first pair slot of the matrix (P / 2 slots)
   int32_t pad;
+  int64_t gram0;      // first partial-Gram slot (P / 2 * ng slots)
 };
 
 struct SvdItem {
@@ -111,9 +122,12 @@ struct SvdItem {
   int32_t k;          // pair index within a round
 };
 
-struct JRot {
-  int32_t p, q;
-  double c, s;
+// One row group of one pair: rows g * kRG .. of X (kind 0) or of V (kind 1).
+struct SvdRowItem {
+  int32_t mat;
+  int32_t k;
+  int32_t g;
+  int32_t kind;
 };
 
 // Circle-method pairing: `players` even, round r in [0, players - 1).
@@ -192,44 +206,47 @@ __global__ void __launch_bounds__(kThreads) k_svd_sort(const SvdMat *mats, const
   for (int64_t c = threadIdx.x; c < mt.N; c += kThreads) perm[mt.p_off + c] = so[c];
 }
 
-template <bool C>
-__global__ void __launch_bounds__(kThreads) k_svd_round(const SvdMat *mats, const SvdItem *items, int32_t round,
-                                                         void *ws_, const int32_t *perm, const int32_t *active,
-                                                         int32_t *rotated) {
-  using T = typename ElemOf<C>::T;
-  __shared__ T sP[kRC][kW + 1];
-  __shared__ T sG[kW][kW + 1];
-  __shared__ T sQ[kW][kW + 1];
-  __shared__ JRot sRot[kW / 2];
-  __shared__ T sEm[kW / 2];
-  __shared__ int32_t sCol[kW];
-  __shared__ double sRed[kWaves];
-  __shared__ int32_t sAny;
-
-  const SvdItem it = items[blockIdx.x];
-  if (!active[it.mat]) return;
-  const SvdMat mt = mats[it.mat];
-  if (round >= mt.P - 1) return;
+// The pair (bi, bj) of round `round`, its column count w and its columns.
+// Returns false when the matrix sits this round out or the pair has < 2 columns.
+__device__ __forceinline__ bool pair_cols(const SvdMat &mt, int32_t round, int32_t k, const int32_t *perm,
+                                          int32_t *sCol, int &w) {
+  if (round >= mt.P - 1) return false;
   int bi, bj;
-  circle_pair(mt.P, round, it.k, bi, bj);
-  const int tid = threadIdx.x;
-  const int32_t *pm = perm + mt.p_off;
+  circle_pair(mt.P, round, k, bi, bj);
   const int wi = bi < mt.nb ? static_cast<int>(min<int64_t>(kB, mt.N - int64_t(bi) * kB)) : 0;
   const int wj = bj < mt.nb ? static_cast<int>(min<int64_t>(kB, mt.N - int64_t(bj) * kB)) : 0;
-  const int w = wi + wj;
-  if (w < 2) return;
+  w = wi + wj;
+  if (w < 2) return false;
+  const int32_t *pm = perm + mt.p_off;
+  const int tid = threadIdx.x;
   if (tid < w) sCol[tid] = tid < wi ? pm[int64_t(bi) * kB + tid] : pm[int64_t(bj) * kB + (tid - wi)];
-  T *X = static_cast<T *>(ws_) + mt.x_off;
-  T *V = static_cast<T *>(ws_) + mt.v_off;
-  __syncthreads();
+  return true;
+}
 
-  // ---- 1. Gram matrix of the panel ----
+// Round phase 1: the partial Gram matrix P^H P of one kRG-row group of the
+// pair's panel P = [X_I X_J], stored w x w with row stride kW.
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_gram(const SvdMat *mats, const SvdRowItem *items, int32_t round,
+                                                        const void *ws_, const int32_t *perm, const int32_t *active,
+                                                        void *part_) {
+  using T = typename ElemOf<C>::T;
+  __shared__ T sP[kRC][kW + 1];
+  __shared__ int32_t sCol[kW];
+  const SvdRowItem it = items[blockIdx.x];
+  if (!active[it.mat]) return;
+  const SvdMat mt = mats[it.mat];
+  int w = 0;
+  if (!pair_cols(mt, round, it.k, perm, sCol, w)) return;
+  const int tid = threadIdx.x;
+  const T *X = static_cast<const T *>(ws_) + mt.x_off;
+  __syncthreads();
+  const int64_t rb = int64_t(it.g) * kRG, re = min<int64_t>(mt.M, rb + kRG);
   T acc[4] = {zero_<T>(), zero_<T>(), zero_<T>(), zero_<T>()};
-  for (int64_t r0 = 0; r0 < mt.M; r0 += kRC) {
+  for (int64_t r0 = rb; r0 < re; r0 += kRC) {
     for (int idx = tid; idx < kRC * w; idx += kThreads) {
       const int c = idx / kRC, rr = idx - c * kRC;
       const int64_t row = r0 + rr;
-      sP[rr][c] = row < mt.M ? X[int64_t(sCol[c]) * mt.M + row] : zero_<T>();
+      sP[rr][c] = row < re ? X[int64_t(sCol[c]) * mt.M + row] : zero_<T>();
     }
     __syncthreads();
 #pragma unroll
@@ -245,14 +262,46 @@ __global__ void __launch_bounds__(kThreads) k_svd_round(const SvdMat *mats, cons
     }
     __syncthreads();
   }
+  T *part = static_cast<T *>(part_) + (mt.gram0 + int64_t(it.k) * mt.ng + it.g) * (kW * kW);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int e = tid + u * kThreads;
-    if (e < w * w) sG[e / w][e % w] = acc[u];
+    if (e < w * w) part[(e / w) * kW + e % w] = acc[u];
+  }
+}
+
+// Round phase 2, one workgroup per pair: the Gram matrix as the sum of the
+// row groups' partials (in group order), the pair's convergence test, and the
+// cyclic two-sided Jacobi eigen-solve of the Gram in LDS, whose unitary Q
+// (w x w, row stride kW) phase 3 applies.  qflag = 1 when Q is to be applied.
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, const SvdItem *items, int32_t round,
+                                                         const void *part_, const int32_t *perm,
+                                                         const int32_t *active, void *q_, int32_t *qflag,
+                                                         int32_t *rotated) {
+  using T = typename ElemOf<C>::T;
+  __shared__ T sG[kW][kW + 1];
+  __shared__ T sQ[kW][kW + 1];
+  __shared__ int32_t sCol[kW];
+  __shared__ double sRed[kWaves];
+
+  const SvdItem it = items[blockIdx.x];
+  if (!active[it.mat]) return;
+  const SvdMat mt = mats[it.mat];
+  const int tid = threadIdx.x;
+  int w = 0;
+  if (!pair_cols(mt, round, it.k, perm, sCol, w)) return;
+  int32_t *flag = qflag + mt.pair0 + it.k;
+  const T *part = static_cast<const T *>(part_) + (mt.gram0 + int64_t(it.k) * mt.ng) * (kW * kW);
+  for (int e = tid; e < w * w; e += kThreads) {
+    const int p = e / w, q = e - p * w;
+    T s = zero_<T>();
+    for (int g = 0; g < mt.ng; ++g) s = add_(s, part[int64_t(g) * (kW * kW) + p * kW + q]);
+    sG[p][q] = s;
   }
   __syncthreads();
 
-  // ---- 2. convergence test of the pair ----
+  // ---- convergence test of the pair ----
   double off = 0.0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -270,104 +319,155 @@ __global__ void __launch_bounds__(kThreads) k_svd_round(const SvdMat *mats, cons
   __syncthreads();
   const double tol = fmax(16.0, static_cast<double>(mt.M)) * 2.220446049250313e-16;
   const double offmax = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
-  if (offmax < tol * tol) return;
-  if (tid == 0) rotated[it.mat] = 1;
+  if (offmax < tol * tol) {
+    if (tid == 0) *flag = 0;
+    return;
+  }
+  if (tid == 0) {
+    rotated[it.mat] = 1;
+    *flag = 1;
+  }
 
-  // ---- 3. Jacobi diagonalisation of G in LDS ----
+  // ---- Jacobi diagonalisation of G in LDS ----
+  // A round applies the rotations of P2 / 2 disjoint index pairs.  Thread
+  // tid updates entries of pairs tid / kW and tid / kW + 8 only (the column
+  // pass covers G and Q, kW entries per pair each; the row pass covers G), so
+  // it computes those two pairs' rotations itself from the G of the previous
+  // round: two barriers per round.
   for (int e = tid; e < w * w; e += kThreads) {
     const int p = e / w, q = e - p * w;
     sQ[p][q] = (p == q) ? one_<T>() : zero_<T>();
   }
   const int P2 = w + (w & 1);
   const double eps = 2.220446049250313e-16;
+  static_assert(kW == 32 && kThreads == 256, "thread-to-pair map of the Jacobi rounds");
+  const int i_el = tid % kW;              // row (column pass) / column (row pass) of this thread's entries
+  __syncthreads();
   for (int sweep = 0; sweep < kInnerSweeps; ++sweep) {
-    if (tid == 0) sAny = 0;
-    __syncthreads();
+    int any = 0;
     for (int r = 0; r < P2 - 1; ++r) {
-      if (tid < P2 / 2) {
-        int p, q;
-        circle_pair(P2, r, tid, p, q);
-        JRot rt{-1, -1, 1.0, 0.0};
-        T em = one_<T>();
-        if (p < w && q < w) {
-          const double al = re_(sG[p][p]), be = re_(sG[q][q]);
-          const T g = sG[p][q];
-          const double ag = sqrt(abs2_(g));
-          if (ag > 0.0 && al > 0.0 && be > 0.0 && ag > eps * sqrt(al * be)) {
-            const double zeta = (be - al) / (2.0 * ag);
-            double t;
-            if (fabs(zeta) > 1e150) t = 0.5 / zeta;
-            else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            const double c = 1.0 / sqrt(1.0 + t * t);
-            rt = JRot{p, q, c, c * t};
-            em = uphase_conj(g, ag);
-            sAny = 1;
+      // rotations of this thread's two pairs, from the current G
+      int pp[2], qq[2];
+      double cc[2], ss[2];
+      T em[2];
+      bool on[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = tid / kW + 8 * h;
+        on[h] = false;
+        cc[h] = 1.0;
+        ss[h] = 0.0;
+        em[h] = one_<T>();
+        pp[h] = qq[h] = 0;
+        if (k < P2 / 2) {
+          int p, q;
+          circle_pair(P2, r, k, p, q);
+          pp[h] = p;
+          qq[h] = q;
+          if (p < w && q < w) {
+            const double al = re_(sG[p][p]), be = re_(sG[q][q]);
+            const T g = sG[p][q];
+            const double ag = sqrt(abs2_(g));
+            if (ag > 0.0 && al > 0.0 && be > 0.0 && ag > eps * sqrt(al * be)) {
+              const double zeta = (be - al) / (2.0 * ag);
+              double t;
+              if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+              else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+              cc[h] = 1.0 / sqrt(1.0 + t * t);
+              ss[h] = cc[h] * t;
+              em[h] = uphase_conj(g, ag);
+              on[h] = true;
+              any = 1;
+            }
           }
         }
-        sRot[tid] = rt;
-        sEm[tid] = em;
       }
+      // every thread has read the G entries its rotations need
       __syncthreads();
       // columns of G and Q:  [x_p, x_q] <- [x_p, x_q] J,
       // J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
-      for (int idx = tid; idx < (P2 / 2) * w * 2; idx += kThreads) {
-        const int which = idx / ((P2 / 2) * w);
-        const int rem = idx - which * (P2 / 2) * w;
-        const int k = rem / w, i = rem - k * w;
-        const JRot rt = sRot[k];
-        if (rt.p < 0) continue;
-        const T em = sEm[k];
-        T(*Mx)[kW + 1] = which ? sQ : sG;
-        const T xp = Mx[i][rt.p], xq = Mx[i][rt.q];
-        const T emxq = mul_(em, xq);
-        Mx[i][rt.p] = sub_(scl_(rt.c, xp), scl_(rt.s, emxq));
-        Mx[i][rt.q] = add_(scl_(rt.s, xp), scl_(rt.c, emxq));
-      }
-      __syncthreads();
-      // rows of G:  [g_p; g_q] <- J^H [g_p; g_q]
-      for (int idx = tid; idx < (P2 / 2) * w; idx += kThreads) {
-        const int k = idx / w, j = idx - k * w;
-        const JRot rt = sRot[k];
-        if (rt.p < 0) continue;
-        const T ep = conj_(sEm[k]);
-        const T gp = sG[rt.p][j], gq = sG[rt.q][j];
-        const T epgq = mul_(ep, gq);
-        T np = sub_(scl_(rt.c, gp), scl_(rt.s, epgq));
-        T nq = add_(scl_(rt.s, gp), scl_(rt.c, epgq));
-        if (j == rt.q) np = zero_<T>();
-        if (j == rt.p) nq = zero_<T>();
-        sG[rt.p][j] = np;
-        sG[rt.q][j] = nq;
-      }
-      __syncthreads();
-    }
-    if (!sAny) break;
-    __syncthreads();
-  }
-  __syncthreads();
-
-  // ---- 4. P <- P Q and V_panel <- V_panel Q ----
-  for (int pass = 0; pass < 2; ++pass) {
-    T *base = pass ? V : X;
-    const int64_t rows = pass ? mt.N : mt.M;
-    for (int64_t r0 = 0; r0 < rows; r0 += kRC) {
-      for (int idx = tid; idx < kRC * w; idx += kThreads) {
-        const int c = idx / kRC, rr = idx - c * kRC;
-        const int64_t row = r0 + rr;
-        sP[rr][c] = row < rows ? base[int64_t(sCol[c]) * rows + row] : zero_<T>();
-      }
-      __syncthreads();
-      for (int idx = tid; idx < kRC * w; idx += kThreads) {
-        const int j = idx / kRC, rr = idx - j * kRC;
-        const int64_t row = r0 + rr;
-        if (row < rows) {
-          T s = zero_<T>();
-          for (int c = 0; c < w; ++c) s = add_(s, mul_(sP[rr][c], sQ[c][j]));
-          base[int64_t(sCol[j]) * rows + row] = s;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!on[h] || i_el >= w) continue;
+#pragma unroll
+        for (int which = 0; which < 2; ++which) {
+          T(*Mx)[kW + 1] = which ? sQ : sG;
+          const T xp = Mx[i_el][pp[h]], xq = Mx[i_el][qq[h]];
+          const T emxq = mul_(em[h], xq);
+          Mx[i_el][pp[h]] = sub_(scl_(cc[h], xp), scl_(ss[h], emxq));
+          Mx[i_el][qq[h]] = add_(scl_(ss[h], xp), scl_(cc[h], emxq));
         }
       }
       __syncthreads();
+      // rows of G:  [g_p; g_q] <- J^H [g_p; g_q]
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!on[h] || i_el >= w) continue;
+        const int j = i_el;
+        const T ep = conj_(em[h]);
+        const T gp = sG[pp[h]][j], gq = sG[qq[h]][j];
+        const T epgq = mul_(ep, gq);
+        T np = sub_(scl_(cc[h], gp), scl_(ss[h], epgq));
+        T nq = add_(scl_(ss[h], gp), scl_(cc[h], epgq));
+        if (j == qq[h]) np = zero_<T>();
+        if (j == pp[h]) nq = zero_<T>();
+        sG[pp[h]][j] = np;
+        sG[qq[h]][j] = nq;
+      }
+      __syncthreads();
     }
+    if (!__syncthreads_or(any)) break;
+  }
+  T *Q = static_cast<T *>(q_) + int64_t(mt.pair0 + it.k) * (kW * kW);
+  for (int e = tid; e < w * w; e += kThreads) {
+    const int p = e / w, q = e - p * w;
+    Q[p * kW + q] = sQ[p][q];
+  }
+}
+
+// Round phase 3: P <- P Q on one kRG-row group of the pair's columns of X
+// (kind 0) or of V (kind 1).
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_update(const SvdMat *mats, const SvdRowItem *items, int32_t round,
+                                                          void *ws_, const int32_t *perm, const int32_t *active,
+                                                          const void *q_, const int32_t *qflag) {
+  using T = typename ElemOf<C>::T;
+  __shared__ T sP[kRC][kW + 1];
+  __shared__ T sQ[kW][kW + 1];
+  __shared__ int32_t sCol[kW];
+  const SvdRowItem it = items[blockIdx.x];
+  if (!active[it.mat]) return;
+  const SvdMat mt = mats[it.mat];
+  int w = 0;
+  if (!pair_cols(mt, round, it.k, perm, sCol, w)) return;
+  if (!qflag[mt.pair0 + it.k]) return;
+  const int tid = threadIdx.x;
+  const T *Q = static_cast<const T *>(q_) + int64_t(mt.pair0 + it.k) * (kW * kW);
+  for (int e = tid; e < w * w; e += kThreads) {
+    const int p = e / w, q = e - p * w;
+    sQ[p][q] = Q[p * kW + q];
+  }
+  T *base = static_cast<T *>(ws_) + (it.kind ? mt.v_off : mt.x_off);
+  const int64_t rows = it.kind ? mt.N : mt.M;
+  const int64_t rb = int64_t(it.g) * kRG, re = min<int64_t>(rows, rb + kRG);
+  __syncthreads();
+  for (int64_t r0 = rb; r0 < re; r0 += kRC) {
+    for (int idx = tid; idx < kRC * w; idx += kThreads) {
+      const int c = idx / kRC, rr = idx - c * kRC;
+      const int64_t row = r0 + rr;
+      sP[rr][c] = row < re ? base[int64_t(sCol[c]) * rows + row] : zero_<T>();
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kRC * w; idx += kThreads) {
+      const int j = idx / kRC, rr = idx - j * kRC;
+      const int64_t row = r0 + rr;
+      if (row < re) {
+        T s = zero_<T>();
+        for (int c = 0; c < w; ++c) s = add_(s, mul_(sP[rr][c], sQ[c][j]));
+        base[int64_t(sCol[j]) * rows + row] = s;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -569,8 +669,9 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
   QLT_CHECK_ARG(n_mats <= 65535, "qlt_block_svd: more than 65535 matrices in one call");
   std::vector<SvdMat> mats(static_cast<size_t>(n_mats));
   std::vector<SvdItem> items;
-  int64_t ws_elems = 0, perm_elems = 0, max_elems = 0;
-  int32_t max_players = 0;
+  std::vector<SvdRowItem> gitems, uitems;
+  int64_t ws_elems = 0, perm_elems = 0, max_elems = 0, gram_slots = 0;
+  int32_t max_players = 0, pair_slots = 0;
   for (int64_t i = 0; i < n_mats; ++i) {
     SvdMat &mt = mats[static_cast<size_t>(i)];
     mt.m = rows[i];
@@ -592,23 +693,52 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
     mt.vt_off = vt_offsets[i];
     mt.nb = static_cast<int32_t>((mt.N + kB - 1) / kB);
     mt.P = mt.nb == 1 ? 2 : mt.nb + (mt.nb & 1);
+    mt.ng = static_cast<int32_t>((mt.M + kRG - 1) / kRG);
+    mt.pair0 = pair_slots;
+    mt.pad = 0;
+    mt.gram0 = gram_slots;
     max_elems = std::max(max_elems, mt.M * mt.N);
     if (mt.N >= 2) {
       max_players = std::max(max_players, mt.P);
-      for (int32_t k = 0; k < mt.P / 2; ++k) items.push_back(SvdItem{static_cast<int32_t>(i), k});
+      const int32_t ngv = static_cast<int32_t>((mt.N + kRG - 1) / kRG);
+      for (int32_t k = 0; k < mt.P / 2; ++k) {
+        items.push_back(SvdItem{static_cast<int32_t>(i), k});
+        for (int32_t g = 0; g < mt.ng; ++g) {
+          gitems.push_back(SvdRowItem{static_cast<int32_t>(i), k, g, 0});
+          uitems.push_back(SvdRowItem{static_cast<int32_t>(i), k, g, 0});
+        }
+        for (int32_t g = 0; g < ngv; ++g) uitems.push_back(SvdRowItem{static_cast<int32_t>(i), k, g, 1});
+      }
+      pair_slots += mt.P / 2;
+      gram_slots += int64_t(mt.P / 2) * mt.ng;
     }
   }
-  DevBuf d_ws, d_mats, d_items, d_perm, d_flags;
+  QLT_CHECK_ARG(gitems.size() < (size_t(1) << 31) && uitems.size() < (size_t(1) << 31),
+                "qlt_block_svd: too many row groups in one call");
+  DevBuf d_ws, d_mats, d_items, d_perm, d_flags, d_gitems, d_uitems, d_part, d_q, d_qflag;
   QLT_HIP(hipMalloc(&d_ws.p, std::max<size_t>(1, static_cast<size_t>(ws_elems) * esz)));
   QLT_HIP(hipMalloc(&d_mats.p, sizeof(SvdMat) * mats.size()));
   QLT_HIP(hipMalloc(&d_items.p, sizeof(SvdItem) * std::max<size_t>(1, items.size())));
+  QLT_HIP(hipMalloc(&d_gitems.p, sizeof(SvdRowItem) * std::max<size_t>(1, gitems.size())));
+  QLT_HIP(hipMalloc(&d_uitems.p, sizeof(SvdRowItem) * std::max<size_t>(1, uitems.size())));
+  QLT_HIP(hipMalloc(&d_part.p, esz * kW * kW * static_cast<size_t>(std::max<int64_t>(1, gram_slots))));
+  QLT_HIP(hipMalloc(&d_q.p, esz * kW * kW * static_cast<size_t>(std::max<int32_t>(1, pair_slots))));
+  QLT_HIP(hipMalloc(&d_qflag.p, sizeof(int32_t) * static_cast<size_t>(std::max<int32_t>(1, pair_slots))));
   QLT_HIP(hipMalloc(&d_perm.p, sizeof(int32_t) * std::max<int64_t>(1, perm_elems)));
   QLT_HIP(hipMalloc(&d_flags.p, sizeof(int32_t) * 2 * mats.size()));
   auto *active = static_cast<int32_t *>(d_flags.p);
   auto *rotated = active + mats.size();
   QLT_HIP(hipMemcpyAsync(d_mats.p, mats.data(), sizeof(SvdMat) * mats.size(), hipMemcpyHostToDevice, st));
-  if (!items.empty())
+  if (!items.empty()) {
     QLT_HIP(hipMemcpyAsync(d_items.p, items.data(), sizeof(SvdItem) * items.size(), hipMemcpyHostToDevice, st));
+    QLT_HIP(hipMemcpyAsync(d_gitems.p, gitems.data(), sizeof(SvdRowItem) * gitems.size(), hipMemcpyHostToDevice,
+                           st));
+    QLT_HIP(hipMemcpyAsync(d_uitems.p, uitems.data(), sizeof(SvdRowItem) * uitems.size(), hipMemcpyHostToDevice,
+                           st));
+  }
+  const auto *gi = static_cast<const SvdRowItem *>(d_gitems.p);
+  const auto *ui = static_cast<const SvdRowItem *>(d_uitems.p);
+  auto *qflag = static_cast<int32_t *>(d_qflag.p);
   const auto *dm = static_cast<const SvdMat *>(d_mats.p);
   auto *perm = static_cast<int32_t *>(d_perm.p);
   const unsigned gx = static_cast<unsigned>(std::min<int64_t>(1024, std::max<int64_t>(1, (max_elems + kThreads - 1) / kThreads)));
@@ -625,9 +755,14 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
     QLT_HIP(hipMemcpyAsync(rotated, h_rot.data(), sizeof(int32_t) * mats.size(), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_svd_sort<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm,
                        d_ws.p, perm, active);
-    for (int32_t r = 0; r < max_players - 1; ++r)
-      hipLaunchKernelGGL(k_svd_round<C>, dim3(static_cast<unsigned>(items.size())), dim3(kThreads), 0, st, dm,
-                         static_cast<const SvdItem *>(d_items.p), r, d_ws.p, perm, active, rotated);
+    for (int32_t r = 0; r < max_players - 1; ++r) {
+      hipLaunchKernelGGL(k_svd_gram<C>, dim3(static_cast<unsigned>(gitems.size())), dim3(kThreads), 0, st, dm, gi,
+                         r, d_ws.p, perm, active, d_part.p);
+      hipLaunchKernelGGL(k_svd_solve<C>, dim3(static_cast<unsigned>(items.size())), dim3(kThreads), 0, st, dm,
+                         static_cast<const SvdItem *>(d_items.p), r, d_part.p, perm, active, d_q.p, qflag, rotated);
+      hipLaunchKernelGGL(k_svd_update<C>, dim3(static_cast<unsigned>(uitems.size())), dim3(kThreads), 0, st, dm, ui,
+                         r, d_ws.p, perm, active, d_q.p, qflag);
+    }
     QLT_HIP(hipGetLastError());
     QLT_HIP(hipMemcpyAsync(h_rot.data(), rotated, sizeof(int32_t) * mats.size(), hipMemcpyDeviceToHost, st));
     QLT_HIP(hipStreamSynchronize(st));

@@ -208,9 +208,10 @@ class SVDResult:
 
 
 def svd(t: BlockSparseTensor, ldims: int, lqndiv, trunc_err: float, dmin: int, dmax: int,
-        t_dev: "capi.DeviceBuffer | None" = None, stream=None) -> SVDResult:
+        t_dev: "capi.DeviceBuffer | None" = None, stream=None, to_host: bool = True) -> SVDResult:
     """Truncated block SVD (SVD, ten_svd.h:241-263).  The U, S, Vt data are
-    returned on the host (``.data``) and kept on the device (``.dev``)."""
+    kept on the device (``.dev``) and, with ``to_host``, also copied to the
+    host (``.data``; otherwise U.data and Vt.data are None)."""
     assert t.rank >= 2 and 1 <= ldims < t.rank and dmin <= dmax and trunc_err >= 0
     mats = block_matrices(t, ldims, lqndiv)
     if t_dev is None:
@@ -271,8 +272,11 @@ def svd(t: BlockSparseTensor, ldims: int, lqndiv, trunc_err: float, dmin: int, d
         sdata[S.offsets[b]:S.offsets[b] + k * k].reshape(k, k)[np.arange(k), np.arange(k)] = sv
     S.data = sdata
     S.dev = capi.DeviceBuffer.from_numpy(sdata if sdata.size else np.zeros(1))
-    U.data = U.dev.to_numpy(U.np_dtype, U.raw_size)
-    Vt.data = Vt.dev.to_numpy(Vt.np_dtype, Vt.raw_size)
+    if to_host:
+        U.data = U.dev.to_numpy(U.np_dtype, U.raw_size)
+        Vt.data = Vt.dev.to_numpy(Vt.np_dtype, Vt.raw_size)
+    else:
+        capi.check(capi.lib().qlt_stream_sync(stream))
     return SVDResult(U, S, Vt, err, D, int(sweeps.value), svals)
 
 
