@@ -68,19 +68,20 @@ def test_other_buckets_keep_the_builtin_rule(builtin_tuning):
 
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_unsliced_target_keeps_one_tile_per_output_tile(builtin_tuning, dtype):
-    # one 64 x 64 output per block with k = 2048 (128 steps): the built-in
+    # one 64 x 64 output per block with k = 2048 (128 steps), which the
+    # built-in rule covers with two 32 x 64 tiles (class 1): the built-in
     # target cuts each tile into k-slices; -1 keeps them whole
     _say(f"case: slice target, 4 blocks 64x2048x64 {dtype}")
     a, b = _diag_pair(64, 2048, 64, 4, dtype)
     ref = O.OraclePlan(a, b, [1], [0])
     exp = ref.execute()
     plan0, c0 = gpu_contract(a, b, [1], [0])
-    assert plan0.info.n_tiles > 4
+    assert plan0.info.n_tiles > 8
     d = T.get_tuning()
-    d["slice_target"][0] = -1
+    d["slice_target"][1] = -1
     T.set_tuning(d)
     plan1, c1 = gpu_contract(a, b, [1], [0])
-    assert plan1.info.n_tiles == 4
+    assert plan1.info.n_tiles == 8
     assert_plan_equal(plan1, ref)
     assert per_block_rel_err(c0, exp) <= TOL
     assert per_block_rel_err(c1, exp) <= TOL
