@@ -95,6 +95,29 @@ def test_values_random_wide_blocks(qlt, seed, dtype):
     assert per_block_rel_err(c, ref.execute()) <= TOL
 
 
+@pytest.mark.parametrize("scale", [1e-8, 1e8])
+@pytest.mark.parametrize("mode", [capi.QLT_MODE_NATIVE, capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8])
+def test_complex_parts_of_very_different_size(qlt, scale, mode):
+    """Complex128 blocks whose imaginary parts are 1e-8 (or 1e8) times their
+    real parts, in A and in B: the native kernel's three-product form builds
+    im = (ar+ai)(br+bi) - ar br - ai bi, whose rounding scales with the larger
+    part, and the split modes share one exponent per row over both parts;
+    every output block stays within 1e-12 of the oracle in Frobenius norm."""
+    rng = np.random.default_rng(9100)
+    a, b, a_ax, b_ax = random_case(rng, dtype="c128", nq=1, max_rank=3, max_sectors=3, max_deg=40, min_deg=8,
+                                   allow_scalar=False)
+    a.data = a.data.real + 1j * scale * a.data.imag
+    b.data = b.data.real + 1j * scale * b.data.imag
+    _say(f"case: unbalanced complex parts scale={scale} mode={mode}")
+    old = capi.lib().qlt_get_split_min_mean_k()
+    capi.check(capi.lib().qlt_set_split_min_mean_k(0.0))
+    try:
+        plan, c = gpu_contract(a, b, a_ax, b_ax, mode)
+    finally:
+        capi.check(capi.lib().qlt_set_split_min_mean_k(old))
+    assert per_block_rel_err(c, O.OraclePlan(a, b, a_ax, b_ax).execute()) <= TOL
+
+
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_large_blocks_and_ragged_edges(qlt, dtype):
     """Blocks bigger than one 64x64 tile with ragged edges and long k."""
