@@ -517,6 +517,7 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restr
 // accumulates its n outputs with FP64 FMAs.  Bound by reading A and writing
 // C once.
 constexpr int RK = 16;            // k per stage
+constexpr int kRowsLoadGroup = 16;   // A loads in flight per thread
 
 template <bool CPLX, int NMAX, int RPT, typename E>
 __device__ __forceinline__ void rows_epilogue(const Seg &seg, int r0, int tid, const double (&acc)[RPT][NMAX],
@@ -623,32 +624,45 @@ __global__ __launch_bounds__(256) void k_rows_f64(
     }
     __syncthreads();
     const int kc = min(RK, et.k - kk);
-#pragma unroll 2
-    for (int p = 0; p < kc; ++p) {
-      const int32_t co = s_acol[p];
-      double av[RPT], av_i[RPT];
+    // The A loads of G consecutive k steps go out together (G * RPT = 16 per
+    // thread); a step beyond the chunk or a row beyond the block loads the
+    // block's first element and counts zero.
 #pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        const bool ok = roff[q] >= 0;
-        const int64_t e = et.a_off + roff[q] + co;
-        const double2 v = ok ? ld_elem<E, CPLX>(A + e * P) : double2{0.0, 0.0};
-        av[q] = v.x;
-        av_i[q] = v.y;
-      }
+    for (int p0 = 0; p0 < RK; p0 += kRowsLoadGroup / RPT) {
+      constexpr int G = kRowsLoadGroup / RPT;
+      if (p0 < kc) {
+        double av[G][RPT], av_i[G][RPT];
 #pragma unroll
-      for (int j = 0; j < NMAX; ++j) {
-        if (j < n) {
-          const double b = Bs[0][p][j];
-          const double b_im = Bs[P - 1][p][j];
+        for (int g = 0; g < G; ++g) {
+          const int32_t co = s_acol[p0 + g];
 #pragma unroll
           for (int q = 0; q < RPT; ++q) {
-            if (CPLX) {
-              acc[q][j] = fma(av[q], b, acc[q][j]);
-              acc[q][j] = fma(-av_i[q], b_im, acc[q][j]);
-              acc_i[q][j] = fma(av[q], b_im, acc_i[q][j]);
-              acc_i[q][j] = fma(av_i[q], b, acc_i[q][j]);
-            } else {
-              acc[q][j] = fma(av[q], b, acc[q][j]);
+            const bool ok = roff[q] >= 0 && co >= 0;
+            const double2 v = ld_elem<E, CPLX>(A + (ok ? et.a_off + roff[q] + co : et.a_off) * P);
+            av[g][q] = ok ? v.x : 0.0;
+            av_i[g][q] = ok ? v.y : 0.0;
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (p0 + g >= kc) continue;
+          const int p = p0 + g;
+#pragma unroll
+          for (int j = 0; j < NMAX; ++j) {
+            if (j < n) {
+              const double b = Bs[0][p][j];
+              const double b_im = Bs[P - 1][p][j];
+#pragma unroll
+              for (int q = 0; q < RPT; ++q) {
+                if (CPLX) {
+                  acc[q][j] = fma(av[g][q], b, acc[q][j]);
+                  acc[q][j] = fma(-av_i[g][q], b_im, acc[q][j]);
+                  acc_i[q][j] = fma(av[g][q], b_im, acc_i[q][j]);
+                  acc_i[q][j] = fma(av_i[g][q], b, acc_i[q][j]);
+                } else {
+                  acc[q][j] = fma(av[g][q], b, acc[q][j]);
+                }
+              }
             }
           }
         }

@@ -208,19 +208,26 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
       s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
     }
     __syncthreads();
+    // the 16 loads go out together: an element outside the block reads row
+    // r0, k p0 (inside it) and is not counted
+    const int64_t e0 = base + s_roff[0] + s_coff[0];
     if (kfast) {
       const int32_t co = s_coff[lane];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int32_t ro = s_roff[wv + 4 * q];
-        if (ro >= 0 && co >= 0) mx[q] = fmax(mx[q], absmax(base + ro + co));
+        const bool ok = ro >= 0 && co >= 0;
+        const double x = absmax(ok ? base + ro + co : e0);
+        mx[q] = fmax(mx[q], ok ? x : 0.0);   // the value is used either way, so the load is not sunk into a branch
       }
     } else {
       const int32_t ro = s_roff[lane];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int32_t co = s_coff[wv + 4 * q];
-        if (ro >= 0 && co >= 0) mx[0] = fmax(mx[0], absmax(base + ro + co));
+        const bool ok = ro >= 0 && co >= 0;
+        const double x = absmax(ok ? base + ro + co : e0);
+        mx[0] = fmax(mx[0], ok ? x : 0.0);
       }
     }
     __syncthreads();
@@ -262,24 +269,30 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
       s_coff[tid] = (p < pm.cols) ? col_off(p) : -1;
     }
     __syncthreads();
-    for (int q = 0; q < (64 * KP) / 256; ++q) {
-      int r, p;
-      if (kfast) {
-        p = tid % KP;
-        r = tid / KP + (256 / KP) * q;
-      } else {
-        r = tid & 63;
-        p = (tid >> 6) + 4 * q;
-      }
+    // all loads go out before the first LDS store: an element outside the
+    // block reads row r0, k p0 (inside it) and is stored as zero
+    constexpr int NQ = (64 * KP) / 256;
+    using Elem = std::conditional_t<CPLX, double2, double>;
+    const int64_t e0 = base + s_roff[0] + s_coff[0];
+    Elem xs[NQ];
+    bool oks[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int r = kfast ? tid / KP + (256 / KP) * q : tid & 63;
+      const int p = kfast ? tid % KP : (tid >> 6) + 4 * q;
       const int32_t ro = s_roff[r], co = s_coff[p];
-      const bool ok = (ro >= 0) && (co >= 0);
-      const int64_t e = base + ro + co;
-      if (CPLX) {
-        const double2 v = ok ? reinterpret_cast<const double2 *>(src)[e] : double2{0.0, 0.0};
-        s_tile[0][r][p] = v.x;
-        s_tile[PARTS - 1][r][p] = v.y;
+      oks[q] = (ro >= 0) && (co >= 0);
+      xs[q] = reinterpret_cast<const Elem *>(src)[oks[q] ? base + ro + co : e0];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int r = kfast ? tid / KP + (256 / KP) * q : tid & 63;
+      const int p = kfast ? tid % KP : (tid >> 6) + 4 * q;
+      if constexpr (CPLX) {
+        s_tile[0][r][p] = oks[q] ? xs[q].x : 0.0;
+        s_tile[PARTS - 1][r][p] = oks[q] ? xs[q].y : 0.0;
       } else {
-        s_tile[0][r][p] = ok ? src[e] : 0.0;
+        s_tile[0][r][p] = oks[q] ? xs[q] : 0.0;
       }
     }
     __syncthreads();
@@ -375,14 +388,15 @@ __global__ __launch_bounds__(256) void k_pack(const PackItem *__restrict__ items
   }
 }
 
-// Single-read pack of the blocks whose k_pad is at most kShortK: one
-// workgroup of 256 threads per (side, block, group of RW rows) stages the
-// rows' whole k extent in LDS (the block is read once), then the KW / 8
-// threads of a row take the row maximum and split 8 consecutive k each.  KW
-// (64, 128, 256, 512 or 1024, at least k_pad; carried in the item's k0 field)
-// and RW = 2048 / KW keep every lane busy on short blocks; a row of 1024 k
-// spans two waves, whose maxima meet in LDS.  LDS rows hold one pad double
-// per 8 so that the 8-element runs are read without bank conflicts.
+// Single-read pack of the blocks whose k_pad is at most kShortK.  One
+// workgroup of 256 threads per (side, block, group of RW real rows) stages
+// the rows' whole k extent in LDS (the block is read once), then the
+// TPR = k_pad / 8 consecutive threads of a row take the row maximum and split
+// 8 consecutive k each.  RW = floor(256 / TPR), so at least half and usually
+// nearly all lanes carry data; a row may straddle waves.  The zero padding
+// of the planes (rows R .. R_pad) and of the exponents is written once, when
+// the workspace is prepared, and never touched here.  LDS rows hold one pad
+// double per 8 so that the 8-element runs are read without bank conflicts.
 constexpr int kShortK = 1024;
 template <bool CPLX, bool I8>
 __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__ items,
@@ -398,79 +412,103 @@ __global__ __launch_bounds__(256) void k_pack_short(const PackItem *__restrict__
   constexpr int RUN = 8;                         // consecutive k per thread
   constexpr int ELEMS = 256 * RUN;               // elements per part per workgroup
   constexpr int KCF = I8 ? KC8 : KC;
-  __shared__ int32_t s_roff[ELEMS / 64];
+  constexpr int kMaxRows = 256 / (KC / RUN);     // k_pad >= KC
+  __shared__ int32_t s_roff[kMaxRows];
   __shared__ int32_t s_coff[kShortK];
-  __shared__ double s_tile[PARTS][ELEMS + ELEMS / 8 + 32];
-  __shared__ double s_wmax[4];
+  __shared__ double s_tile[PARTS][ELEMS + ELEMS / 8 + kMaxRows];
+  __shared__ double s_lead[256];
 
   const PackItem it = items[blockIdx.x];
-  const int KW = it.k0, RW = ELEMS / KW, TPR = KW / RUN, lrow = KW + KW / 8 + 1;
   const bool is_a = (it.side == 0);
   const PackMeta pm = is_a ? meta_a[it.blk] : meta_b[it.blk];
   const PackView v = pack_view(is_a, pm, is_a ? ga[it.blk] : gb[it.blk], is_a ? a_offs[it.blk] : b_offs[it.blk], L,
                                is_a ? A : B);
+  const int KP = pm.k_pad, TPR = KP / RUN, RW = 256 / TPR, lrow = KP + KP / 8 + 1;
+  const int nrows = min(RW, pm.rows - it.r0);    // >= 1
   const int tid = threadIdx.x;
-  if (tid < RW) {
-    const int r = it.r0 + tid;
-    s_roff[tid] = (r < pm.rows) ? v.row_off(r) : -1;
-  }
-  for (int p = tid; p < KW; p += 256) s_coff[p] = (p < pm.cols) ? v.col_off(p) : -1;
+  if (tid < nrows) s_roff[tid] = v.row_off(it.r0 + tid);
+  for (int p = tid; p < KP; p += 256) s_coff[p] = (p < pm.cols) ? v.col_off(p) : -1;
   __syncthreads();
 
-  // ---- one read of the RW rows (zeros outside the block) ----
+  // ---- one read of the rows: element f = tid + 256 j of the nrows x KP
+  // extent, k-major when consecutive k are consecutive in memory, else
+  // row-major; (q, w) = (f / d, f % d) advance by (256 / d, 256 % d).  All
+  // eight loads are issued before the first LDS store: an element outside the
+  // block loads the block's first element instead and is stored as zero. ----
+  {
+    const int d = v.kfast ? KP : nrows;
+    const int dq = 256 / d, dw = 256 - dq * d;
+    int q = tid / d, w = tid - q * d;
+    const int n = nrows * KP;
+    const int64_t e0 = v.base + s_roff[0] + s_coff[0];   // row r0, k 0: inside the block
+    using Elem = std::conditional_t<CPLX, double2, double>;
+    Elem xs[RUN];
+    int lps[RUN];
+    bool oks[RUN];
 #pragma unroll
-  for (int j = 0; j < RUN; ++j) {
-    int r, p;
-    if (v.kfast) {
-      const int f = tid + 256 * j;               // KW consecutive k per row
-      r = f / KW;
-      p = f % KW;
-    } else {
-      const int f = tid + 256 * j;               // RW consecutive rows per k
-      r = f % RW;
-      p = f / RW;
+    for (int j = 0; j < RUN; ++j) {
+      const bool inside = tid + 256 * j < n;
+      const int r = inside ? (v.kfast ? q : w) : 0, p = inside ? (v.kfast ? w : q) : 0;
+      const int32_t co = s_coff[p];
+      oks[j] = inside && co >= 0;
+      lps[j] = inside ? r * lrow + p + (p >> 3) : -1;
+      xs[j] = reinterpret_cast<const Elem *>(v.src)[oks[j] ? v.base + s_roff[r] + co : e0];
+      q += dq;
+      w += dw;
+      if (w >= d) {
+        w -= d;
+        ++q;
+      }
     }
-    const int32_t ro = s_roff[r], co = s_coff[p];
-    const bool ok = (ro >= 0) && (co >= 0);
-    const int64_t e = v.base + ro + co;
-    const int lp = r * lrow + p + (p >> 3);
-    if (CPLX) {
-      const double2 x = ok ? reinterpret_cast<const double2 *>(v.src)[e] : double2{0.0, 0.0};
-      s_tile[0][lp] = x.x;
-      s_tile[PARTS - 1][lp] = x.y;
-    } else {
-      s_tile[0][lp] = ok ? v.src[e] : 0.0;
+#pragma unroll
+    for (int j = 0; j < RUN; ++j) {
+      if (lps[j] < 0) continue;
+      if constexpr (CPLX) {
+        s_tile[0][lps[j]] = oks[j] ? xs[j].x : 0.0;
+        s_tile[PARTS - 1][lps[j]] = oks[j] ? xs[j].y : 0.0;
+      } else {
+        s_tile[0][lps[j]] = oks[j] ? xs[j] : 0.0;
+      }
     }
   }
   __syncthreads();
 
-  // ---- row maximum and exponent (TPR threads per row) ----
-  const int r = tid / TPR, run = tid % TPR, p0 = run * RUN;
+  // ---- row maximum and exponent ----
+  const int r = tid / TPR, run = tid - r * TPR, p0 = run * RUN;
+  const bool active = r < nrows;
+  const int lx = (active ? r : 0) * lrow + 9 * run;
   double x[PARTS][RUN];
   double m = 0.0;
 #pragma unroll
   for (int part = 0; part < PARTS; ++part)
 #pragma unroll
     for (int u = 0; u < RUN; ++u) {
-      x[part][u] = s_tile[part][r * lrow + 9 * run + u];
+      x[part][u] = active ? s_tile[part][lx + u] : 0.0;
       m = fmax(m, fabs(x[part][u]));
     }
-  if (TPR <= 64) {
-    for (int o = TPR / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, TPR));
-  } else {                                       // two waves per row
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((tid & 63) == 0) s_wmax[tid >> 6] = m;
-    __syncthreads();
-    m = fmax(s_wmax[2 * r], s_wmax[2 * r + 1]);
+  // Segmented suffix maximum inside the wave: after the step of offset o a
+  // thread holds the maximum over its next 2o threads of the same row (and
+  // wave), so the first thread of a row in each wave ends with the row's
+  // maximum over that wave.
+  const int lane = tid & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_down(m, o, 64);
+    if (run + o < TPR && lane + o < 64) m = fmax(m, y);
   }
+  if (run == 0 || lane == 0) s_lead[tid] = m;
+  __syncthreads();
+  if (!active) return;
+  const int first = r * TPR;                     // the row's threads: first .. first + TPR - 1 < 256
+  double mx = s_lead[first];
+  for (int wv = (first >> 6) + 1; wv <= ((first + TPR - 1) >> 6); ++wv) mx = fmax(mx, s_lead[wv << 6]);
   int ex = 0;
-  if (m > 0.0) (void)frexp(m, &ex);
-  const int grow = it.r0 + r;                    // < r_pad
+  if (mx > 0.0) (void)frexp(mx, &ex);
+  const int grow = it.r0 + r;                    // < rows
   if (run == 0) exps[pm.exp_off + grow] = ex;
-  if (p0 >= pm.k_pad) return;
 
   // ---- split the run p0 .. p0 + 7 (inside one packed k-chunk) ----
-  const int nkc = pm.k_pad / KCF;
+  const int nkc = KP / KCF;
   if constexpr (I8) {
     int8_t *planes = static_cast<int8_t *>(planes_v);
     const int sc = 46 - ex;   // ldexp per element: 2^(46 - e) overflows below 2^-977
@@ -1391,7 +1429,7 @@ struct SplitLayout {
   std::vector<PackMeta> meta_a, meta_b;
   std::vector<PackItem> items;    // phase 1: (side, block, 64-row group)
   std::vector<PackItem> citems;   // phase 2: (side, block, 64-row group, k-chunk)
-  std::vector<PackItem> sitems;   // single-read pack of k_pad <= kShortK: (side, block, row group, k width)
+  std::vector<PackItem> sitems;   // single-read pack of k_pad <= kShortK: (side, block, first row, 0)
   std::vector<SplitTile> rtiles;  // self-contained records of `tiles`
   std::vector<SplitTask> rtasks;  // self-contained records of the plan's tasks
   std::vector<Tile> tiles;        // 64 x 64 output tiles of the MFMA-class blocks, longest first
@@ -1441,9 +1479,9 @@ SplitLayout compute_layout(const qlt_plan *plan, int32_t dtype, int32_t mode) {
       L.plane_elems += static_cast<int64_t>(parts) * fmt.slices * m.r_pad * m.k_pad;
       L.exp_elems += m.r_pad;
       if (m.k_pad <= kShortK) {
-        const int32_t kw = m.k_pad <= 64 ? 64 : m.k_pad <= 128 ? 128 : m.k_pad <= 256 ? 256 : m.k_pad <= 512 ? 512 : 1024;
-        const int32_t rw = 2048 / kw;
-        for (int32_t r0 = 0; r0 < m.r_pad; r0 += rw) L.sitems.push_back(PackItem{side, static_cast<int32_t>(i), r0, kw});
+        // groups of RW = floor(256 / (k_pad / 8)) real rows; the padding rows stay zero
+        const int32_t rw = 256 / (m.k_pad / 8);
+        for (int32_t r0 = 0; r0 < m.rows; r0 += rw) L.sitems.push_back(PackItem{side, static_cast<int32_t>(i), r0, 0});
         continue;
       }
       for (int32_t r0 = 0; r0 < m.r_pad; r0 += 64) {
@@ -1593,6 +1631,11 @@ qlt_status split_prepare_metadata(qlt_plan *plan, int32_t dtype, int32_t mode) {
     aoffs[e.a_blk] = e.a_off;
     boffs[e.b_blk] = e.b_off;
   }
+  // The single-read pack writes the real rows only: the padding rows of the
+  // planes and their exponents are zeroed here, once per layout.
+  QLT_HIP(hipMemset(base + L.off_exps, 0, sizeof(int32_t) * L.exp_elems));
+  QLT_HIP(hipMemset(base + L.off_planes, 0, static_cast<size_t>(split_fmt(mode).elem_bytes) * L.plane_elems));
+  QLT_HIP(hipDeviceSynchronize());
   QLT_HIP(hipMemcpy(base + L.off_meta_a, L.meta_a.data(), sizeof(PackMeta) * L.meta_a.size(),
                     hipMemcpyHostToDevice));
   QLT_HIP(hipMemcpy(base + L.off_meta_b, L.meta_b.data(), sizeof(PackMeta) * L.meta_b.size(),

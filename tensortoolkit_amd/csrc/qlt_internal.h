@@ -379,7 +379,7 @@ struct qlt_plan {
   size_t split_off[12] = {};            // workspace sub-buffer offsets
   int64_t split_tiles = 0;              // 64 x 64 tiles of the split kernel
   int64_t split_citems = 0;             // phase-2 pack work items (row group x k-chunk)
-  int64_t split_sitems = 0;             // single-read pack work items (8-row groups of short blocks)
+  int64_t split_sitems = 0;             // single-read pack work items (row groups of short blocks)
 
   qlt_plan() = default;
   qlt_plan(const qlt_plan &) = delete;
