@@ -124,25 +124,52 @@ class MatvecRunner:
         return self.outs[-1].to_numpy(out.np_dtype, out.raw_size)
 
 
-def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int, probe_step: int):
-    """Times `steps` matvecs.  Inside every step the library's kernel probe
-    brackets the dominant kernel of contraction `probe_step` with its own
-    event pair, recorded on the launch stream."""
+def settle(runner: MatvecRunner, mode: int, seconds: float):
+    """Runs the matvec untimed, in batches of 10, for `seconds` of wall time,
+    so that the GPU clocks have reached their loaded state before warmup (after
+    a pause they take about 20 ms of sustained work to come back up)."""
+    lib = capi.lib()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            for s in range(4):
+                runner.launch(s, mode)
+        capi.check(lib.qlt_stream_sync(runner.stream))
+
+
+def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int, probe_step: int,
+              settle_s: float = 0.0):
+    """Times `steps` matvecs between one event pair on the launch stream:
+    the timed region holds the launches and nothing else.  A second pass of
+    `steps` matvecs after it records an event pair around every contraction
+    (the per-contraction breakdown) and the library's kernel probe around the
+    dominant kernel of contraction `probe_step`; those events add gaps
+    between kernels, so that pass is not the step time."""
     lib = capi.lib()
     for p in runner.plans:
         p.prepare(runner.dtype, mode)
+    settle(runner, mode, settle_s)
     for _ in range(warmup):
         for s in range(4):
             runner.launch(s, mode)
     capi.check(lib.qlt_stream_sync(runner.stream))
-    ev = [[(hip.event(), hip.event()) for _ in range(4)] for _ in range(steps)]
-    kev = [(hip.event(), hip.event()) for _ in range(steps)]
     t_start, t_end = hip.event(), hip.event()
-    name = C.create_string_buffer(96)
-    kflops, kbytes = C.c_double(), C.c_double()
     capi.check(lib.qlt_device_sync())
     wall0 = time.perf_counter()
     hip.record(t_start, runner.stream)
+    for _ in range(steps):
+        for s in range(4):
+            runner.launch(s, mode)
+    hip.record(t_end, runner.stream)
+    capi.check(lib.qlt_stream_sync(runner.stream))
+    capi.check(lib.qlt_device_sync())
+    wall = time.perf_counter() - wall0
+    total_ms = hip.elapsed_ms(t_start, t_end)
+    # breakdown pass
+    ev = [[(hip.event(), hip.event()) for _ in range(4)] for _ in range(steps)]
+    kev = [(hip.event(), hip.event()) for _ in range(steps)]
+    name = C.create_string_buffer(96)
+    kflops, kbytes = C.c_double(), C.c_double()
     for k in range(steps):
         for s in range(4):
             hip.record(ev[k][s][0], runner.stream)
@@ -153,11 +180,7 @@ def time_mode(runner: MatvecRunner, hip: Hip, mode: int, steps: int, warmup: int
                 capi.check(lib.qlt_kernel_probe_last(name, 96, C.byref(kflops), C.byref(kbytes)))
                 capi.check(lib.qlt_kernel_probe_set(None, None))
             hip.record(ev[k][s][1], runner.stream)
-    hip.record(t_end, runner.stream)
     capi.check(lib.qlt_stream_sync(runner.stream))
-    capi.check(lib.qlt_device_sync())
-    wall = time.perf_counter() - wall0
-    total_ms = hip.elapsed_ms(t_start, t_end)
     per_launch = np.array([[hip.elapsed_ms(*ev[k][s]) for s in range(4)] for k in range(steps)])
     kernel = {"name": name.value.decode(), "flops": kflops.value, "bytes": kbytes.value,
               "ms": float(np.mean([hip.elapsed_ms(*kev[k]) for k in range(steps)]))}
@@ -202,7 +225,7 @@ def config2_graph_replay(hip: Hip, stream, replays: int, tune: bool) -> dict:
                            "ms_after": round(res.ms_after, 4), "overrides": TUNE.overrides(res.tuning),
                            "seconds": round(time.perf_counter() - t0, 1)}
         r2.rebuild_plans()
-    total_ms, _, _, _ = time_mode(r2, hip, capi.QLT_MODE_NATIVE, 20, 3, int(np.argmax(r2.flops)))
+    total_ms, _, _, _ = time_mode(r2, hip, capi.QLT_MODE_NATIVE, 20, 3, int(np.argmax(r2.flops)), 0.1)
     graph_ms = time_graph(r2, hip, capi.QLT_MODE_NATIVE, replays)
     out.update(ms_per_step_eager=round(total_ms / 20, 4), gflops_eager=round(flops / (total_ms / 20 * 1e-3) / 1e9, 2),
                ms_per_replay=round(graph_ms, 4), gflops_replay=round(flops / (graph_ms * 1e-3) / 1e9, 2))
@@ -422,6 +445,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle-ms", type=float, default=200.0,
+                    help="untimed run of each mode before its warmup steps, to bring the GPU clocks up")
     ap.add_argument("--config", default="u1u1_D4096", choices=["u1u1_D4096", "u1_D2048", "u1_lognormal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", default="all", choices=["all", "both", "native", "split", "split_bf16", "split_i8"],
@@ -493,7 +518,8 @@ def main() -> int:
                   file=sys.stderr, flush=True)
         runner.rebuild_plans()
         try:
-            total_ms, wall, per_launch, kernel = time_mode(runner, hip, mode, args.steps, args.warmup, dom)
+            total_ms, wall, per_launch, kernel = time_mode(runner, hip, mode, args.steps, args.warmup, dom,
+                                                           args.settle_ms * 1e-3)
         except capi.QltError as e:
             print(f"[bench] mode {name} unavailable: {e}", file=sys.stderr, flush=True)
             continue
@@ -569,6 +595,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_ms": args.settle_ms,
         "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
