@@ -102,6 +102,27 @@ def test_split_pack_widths(forced_split, split_mode, dtype, k):
     assert rel_frobenius(c.data, exp) <= TOL
 
 
+@pytest.mark.parametrize("m", [1, 3])
+@pytest.mark.parametrize("k", [16, 80, 300, 1000])
+@pytest.mark.parametrize("a_transposed", [False, True])
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_pack_few_rows(forced_split, split_mode, dtype, k, m, a_transposed):
+    """A blocks of one to three rows: the single-read pack's workgroup then
+    holds fewer rows than it has room for (up to 128 rows at the narrowest
+    k), in both memory orders of A (k contiguous, or rows contiguous)."""
+    ix_m = Index(((0,),), (m,), IN)
+    ix_k = Index(((0,),), (k,), OUT)
+    ix_n = Index(((0,),), (40,), OUT)
+    legs = [ix_k, ix_m] if a_transposed else [ix_m, ix_k]
+    a = BlockSparseTensor.allowed_blocks(legs, (0,), dtype).fill_uniform(71)
+    b = BlockSparseTensor.allowed_blocks([ix_k.inverse(), ix_n], (0,), dtype).fill_uniform(72)
+    a_ax = [0] if a_transposed else [1]
+    _say(f"case: split few rows mode={split_mode} {dtype} m={m} k={k} a_transposed={a_transposed}")
+    plan, c = gpu_contract(a, b, a_ax, [0], mode=split_mode)
+    exp = O.OraclePlan(a, b, a_ax, [0]).execute()
+    assert rel_frobenius(c.data, exp) <= TOL
+
+
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_split_k_beyond_int8_flush_interval(forced_split, split_mode, dtype):
     """k = 8500 crosses the int8 form's 8192-k flush of its int32 digit
