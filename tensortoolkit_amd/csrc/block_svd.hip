@@ -68,9 +68,9 @@ template <> __device__ __forceinline__ cplx zero_<cplx>() { return {0.0, 0.0}; }
 template <typename T> __device__ __forceinline__ T one_();
 template <> __device__ __forceinline__ double one_<double>() { return 1.0; }
 template <> __device__ __forceinline__ cplx one_<cplx>() { return {1.0, 0.0}; }
-// unit phase of g (|g| > 0), conjugated: e^{-i arg g}
-__device__ __forceinline__ double uphase_conj(double g, double ag) { return g / ag; }
-__device__ __forceinline__ cplx uphase_conj(cplx g, double ag) { return {g.x / ag, -g.y / ag}; }
+// unit phase of g (|g| > 0), conjugated: e^{-i arg g}, from 1 / |g|
+__device__ __forceinline__ double uphase_conj_inv(double g, double inv) { return g * inv; }
+__device__ __forceinline__ cplx uphase_conj_inv(cplx g, double inv) { return {g.x * inv, -g.y * inv}; }
 // complex reciprocal
 __device__ __forceinline__ double recip_(double a) { return 1.0 / a; }
 __device__ __forceinline__ cplx recip_(cplx a) {
@@ -162,18 +162,30 @@ __global__ void __launch_bounds__(kThreads) k_svd_load(const SvdMat *mats, const
     perm[mt.p_off + e] = static_cast<int32_t>(e);
 }
 
-// Squared column norms of X into sn[0..N), then ord[rank] = column by
-// decreasing norm (ties by column index).
-template <typename T>
-__device__ void norms_and_order(const T *X, int64_t M, int64_t N, double *sn, int32_t *ord) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t c = wave; c < N; c += kWaves) {
-    double acc = 0.0;
-    const T *col = X + c * M;
-    for (int64_t i = lane; i < M; i += 64) acc += abs2_(col[i]);
-    acc = wave_sum(acc);
-    if (lane == 0) sn[c] = acc;
-  }
+// Squared column norms of the working matrices, one wave per column:
+// grid (column groups of kWaves, matrices); `all` = 0 skips the matrices
+// that have converged.  norms[p_off + c] = |X_c|^2.
+template <bool C>
+__global__ void __launch_bounds__(kThreads) k_svd_norms(const SvdMat *mats, const void *ws_, const int32_t *active,
+                                                         int32_t all, double *norms) {
+  using T = typename ElemOf<C>::T;
+  const int mi = blockIdx.y;
+  if (!all && !active[mi]) return;
+  const SvdMat mt = mats[mi];
+  const int64_t c = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  if (c >= mt.N) return;
+  const int lane = threadIdx.x & 63;
+  const T *col = static_cast<const T *>(ws_) + mt.x_off + c * mt.M;
+  double acc = 0.0;
+  for (int64_t i = lane; i < mt.M; i += 64) acc += abs2_(col[i]);
+  acc = wave_sum(acc);
+  if (lane == 0) norms[mt.p_off + c] = acc;
+}
+
+// The matrix's squared column norms into sn[0..N), then ord[rank] = column
+// by decreasing norm (ties by column index).
+__device__ void order_by_norms(const double *norms, int64_t N, double *sn, int32_t *ord) {
+  for (int64_t c = threadIdx.x; c < N; c += kThreads) sn[c] = norms[c];
   __syncthreads();
   for (int64_t c = threadIdx.x; c < N; c += kThreads) {
     const double v = sn[c];
@@ -187,22 +199,18 @@ __device__ void norms_and_order(const T *X, int64_t M, int64_t N, double *sn, in
   __syncthreads();
 }
 
-template <bool C>
-__global__ void __launch_bounds__(kThreads) k_svd_sort(const SvdMat *mats, const void *ws_, int32_t *perm,
-                                                        const int32_t *active) {
-  using T = typename ElemOf<C>::T;
+// Column order of the next sweep: the physical columns by decreasing norm
+// (the new order is a permutation of physical columns, so the previous order
+// is not read).
+__global__ void __launch_bounds__(kThreads) k_svd_sort(const SvdMat *mats, const double *norms, int32_t *perm,
+                                                        const int32_t *active, int32_t all) {
   __shared__ double sn[kMaxSortCols];
   __shared__ int32_t so[kMaxSortCols];
   const int mi = blockIdx.x;
-  if (!active[mi]) return;
+  if (!all && !active[mi]) return;
   const SvdMat mt = mats[mi];
   if (mt.N > kMaxSortCols) return;   // keep the previous order
-  const T *X = static_cast<const T *>(ws_) + mt.x_off;
-  const int32_t *old = perm + mt.p_off;
-  // norms in the current physical order; the new order is a permutation of
-  // physical columns, so `old` is not needed beyond its length
-  (void)old;
-  norms_and_order(X, mt.M, mt.N, sn, so);
+  order_by_norms(norms + mt.p_off, mt.N, sn, so);
   for (int64_t c = threadIdx.x; c < mt.N; c += kThreads) perm[mt.p_off + c] = so[c];
 }
 
@@ -280,7 +288,7 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
                                                          const int32_t *active, void *q_, int32_t *qflag,
                                                          int32_t *rotated) {
   using T = typename ElemOf<C>::T;
-  __shared__ T sG[kW][kW + 1];
+  __shared__ T sG[2][kW][kW + 1];   // G of the current round, and the next
   __shared__ T sQ[kW][kW + 1];
   __shared__ int32_t sCol[kW];
   __shared__ double sRed[kWaves];
@@ -297,7 +305,7 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
     const int p = e / w, q = e - p * w;
     T s = zero_<T>();
     for (int g = 0; g < mt.ng; ++g) s = add_(s, part[int64_t(g) * (kW * kW) + p * kW + q]);
-    sG[p][q] = s;
+    sG[0][p][q] = s;
   }
   __syncthreads();
 
@@ -309,8 +317,8 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
     if (e < w * w) {
       const int p = e / w, q = e - p * w;
       if (p < q) {
-        const double dp = re_(sG[p][p]), dq = re_(sG[q][q]);
-        if (dp > 0.0 && dq > 0.0) off = fmax(off, abs2_(sG[p][q]) / (dp * dq));
+        const double dp = re_(sG[0][p][p]), dq = re_(sG[0][q][q]);
+        if (dp > 0.0 && dq > 0.0) off = fmax(off, abs2_(sG[0][p][q]) / (dp * dq));
       }
     }
   }
@@ -329,92 +337,111 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
   }
 
   // ---- Jacobi diagonalisation of G in LDS ----
-  // A round applies the rotations of P2 / 2 disjoint index pairs.  Thread
-  // tid updates entries of pairs tid / kW and tid / kW + 8 only (the column
-  // pass covers G and Q, kW entries per pair each; the row pass covers G), so
-  // it computes those two pairs' rotations itself from the G of the previous
-  // round: two barriers per round.
+  // A round applies the rotations of P2 / 2 disjoint index pairs,
+  // G <- J^H G J and Q <- Q J, as 2 x 2 blocks: thread (bi, bj) takes the
+  // block of G at rows (p, q) of pair bi and columns of pair bj, applies the
+  // rotation of pair bj to its columns and then that of pair bi to its rows
+  // (the same operations, in the same order, as a column pass followed by a
+  // row pass), and writes the result into the other G buffer; it also
+  // rotates rows 2 bi, 2 bi + 1 of Q in the columns of pair bj, in place.
+  // Every thread computes the rotations it needs from the round's G itself,
+  // so a round takes one barrier.
   for (int e = tid; e < w * w; e += kThreads) {
     const int p = e / w, q = e - p * w;
     sQ[p][q] = (p == q) ? one_<T>() : zero_<T>();
   }
   const int P2 = w + (w & 1);
   const double eps = 2.220446049250313e-16;
-  static_assert(kW == 32 && kThreads == 256, "thread-to-pair map of the Jacobi rounds");
-  const int i_el = tid % kW;              // row (column pass) / column (row pass) of this thread's entries
+  static_assert(kW == 32 && kThreads == 256, "one thread per 2 x 2 block of pairs");
+  const int bi = tid / (kW / 2), bj = tid % (kW / 2);
+  const bool blk_on = bi < P2 / 2 && bj < P2 / 2;
+  // rotation of pair k of round r from the round's G: indexes, c, s and the
+  // unit phase; false (identity) when the pair is orthogonal or padded
+  auto rotation = [&](const T (*G)[kW + 1], int r, int k, int &p, int &q, double &c, double &sn, T &em) {
+    circle_pair(P2, r, k, p, q);
+    c = 1.0;
+    sn = 0.0;
+    em = one_<T>();
+    if (p >= w || q >= w) return false;
+    const double al = re_(G[p][p]), be = re_(G[q][q]);
+    const T g = G[p][q];
+    // |g| > eps sqrt(al be), squared: three square roots and three divisions
+    // per rotation, the round's latency chain
+    const double ag2 = abs2_(g);
+    if (!(ag2 > 0.0 && al > 0.0 && be > 0.0 && ag2 > (eps * eps) * (al * be))) return false;
+    const double ag = sqrt(ag2);
+    const double inv = 1.0 / ag;
+    const double zeta = (be - al) * (0.5 * inv);
+    double t;
+    if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+    else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+    c = 1.0 / sqrt(1.0 + t * t);
+    sn = c * t;
+    em = uphase_conj_inv(g, inv);
+    return true;
+  };
   __syncthreads();
+  int cur = 0;
   for (int sweep = 0; sweep < kInnerSweeps; ++sweep) {
     int any = 0;
     for (int r = 0; r < P2 - 1; ++r) {
-      // rotations of this thread's two pairs, from the current G
-      int pp[2], qq[2];
-      double cc[2], ss[2];
-      T em[2];
-      bool on[2];
+      if (blk_on) {
+        const T(*G)[kW + 1] = sG[cur];
+        T(*Gn)[kW + 1] = sG[cur ^ 1];
+        int pi, qi, pj, qj;
+        double ci, si, cj, sj;
+        T emi, emj;
+        const bool oni = rotation(G, r, bi, pi, qi, ci, si, emi);
+        const bool onj = rotation(G, r, bj, pj, qj, cj, sj, emj);
+        any |= oni ? 1 : 0;
+        const int rows[2] = {pi, qi}, cols[2] = {pj, qj};
+        T x[2][2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = tid / kW + 8 * h;
-        on[h] = false;
-        cc[h] = 1.0;
-        ss[h] = 0.0;
-        em[h] = one_<T>();
-        pp[h] = qq[h] = 0;
-        if (k < P2 / 2) {
-          int p, q;
-          circle_pair(P2, r, k, p, q);
-          pp[h] = p;
-          qq[h] = q;
-          if (p < w && q < w) {
-            const double al = re_(sG[p][p]), be = re_(sG[q][q]);
-            const T g = sG[p][q];
-            const double ag = sqrt(abs2_(g));
-            if (ag > 0.0 && al > 0.0 && be > 0.0 && ag > eps * sqrt(al * be)) {
-              const double zeta = (be - al) / (2.0 * ag);
-              double t;
-              if (fabs(zeta) > 1e150) t = 0.5 / zeta;
-              else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-              cc[h] = 1.0 / sqrt(1.0 + t * t);
-              ss[h] = cc[h] * t;
-              em[h] = uphase_conj(g, ag);
-              on[h] = true;
-              any = 1;
-            }
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) x[a][b] = (rows[a] < w && cols[b] < w) ? G[rows[a]][cols[b]] : zero_<T>();
+        // columns:  [x_p, x_q] <- [x_p, x_q] J,  J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
+        if (onj) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const T xp = x[a][0], emxq = mul_(emj, x[a][1]);
+            x[a][0] = sub_(scl_(cj, xp), scl_(sj, emxq));
+            x[a][1] = add_(scl_(sj, xp), scl_(cj, emxq));
+          }
+        }
+        // rows:  [g_p; g_q] <- J^H [g_p; g_q]; the pair's own off-diagonal entries become 0
+        if (oni) {
+          const T ep = conj_(emi);
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const T gp = x[0][b], epgq = mul_(ep, x[1][b]);
+            T np = sub_(scl_(ci, gp), scl_(si, epgq));
+            T nq = add_(scl_(si, gp), scl_(ci, epgq));
+            if (cols[b] == qi) np = zero_<T>();
+            if (cols[b] == pi) nq = zero_<T>();
+            x[0][b] = np;
+            x[1][b] = nq;
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            if (rows[a] < w && cols[b] < w) Gn[rows[a]][cols[b]] = x[a][b];
+        // Q: rows 2 bi, 2 bi + 1, columns of pair bj
+        if (onj) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int i = 2 * bi + a;
+            if (i >= w) continue;
+            const T xp = sQ[i][pj], emxq = mul_(emj, sQ[i][qj]);
+            sQ[i][pj] = sub_(scl_(cj, xp), scl_(sj, emxq));
+            sQ[i][qj] = add_(scl_(sj, xp), scl_(cj, emxq));
           }
         }
       }
-      // every thread has read the G entries its rotations need
       __syncthreads();
-      // columns of G and Q:  [x_p, x_q] <- [x_p, x_q] J,
-      // J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (!on[h] || i_el >= w) continue;
-#pragma unroll
-        for (int which = 0; which < 2; ++which) {
-          T(*Mx)[kW + 1] = which ? sQ : sG;
-          const T xp = Mx[i_el][pp[h]], xq = Mx[i_el][qq[h]];
-          const T emxq = mul_(em[h], xq);
-          Mx[i_el][pp[h]] = sub_(scl_(cc[h], xp), scl_(ss[h], emxq));
-          Mx[i_el][qq[h]] = add_(scl_(ss[h], xp), scl_(cc[h], emxq));
-        }
-      }
-      __syncthreads();
-      // rows of G:  [g_p; g_q] <- J^H [g_p; g_q]
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (!on[h] || i_el >= w) continue;
-        const int j = i_el;
-        const T ep = conj_(em[h]);
-        const T gp = sG[pp[h]][j], gq = sG[qq[h]][j];
-        const T epgq = mul_(ep, gq);
-        T np = sub_(scl_(cc[h], gp), scl_(ss[h], epgq));
-        T nq = add_(scl_(ss[h], gp), scl_(cc[h], epgq));
-        if (j == qq[h]) np = zero_<T>();
-        if (j == pp[h]) nq = zero_<T>();
-        sG[pp[h]][j] = np;
-        sG[qq[h]][j] = nq;
-      }
-      __syncthreads();
+      cur ^= 1;
     }
     if (!__syncthreads_or(any)) break;
   }
@@ -471,43 +498,47 @@ __global__ void __launch_bounds__(kThreads) k_svd_update(const SvdMat *mats, con
   }
 }
 
+// Singular values and vectors, grid (element groups, matrices): the columns
+// in the order of `ord` (decreasing norm, written by k_svd_sort), sigma_k =
+// sqrt(norm of column ord[k]).
 template <bool C>
-__global__ void __launch_bounds__(kThreads) k_svd_finish(const SvdMat *mats, const void *ws_, void *u_, double *s_,
-                                                          void *vt_) {
+__global__ void __launch_bounds__(kThreads) k_svd_finish(const SvdMat *mats, const void *ws_, const double *norms,
+                                                          const int32_t *ord, void *u_, double *s_, void *vt_) {
   using T = typename ElemOf<C>::T;
-  __shared__ double sn[kMaxSortCols];
-  __shared__ int32_t so[kMaxSortCols];
-  const SvdMat mt = mats[blockIdx.x];
+  const SvdMat mt = mats[blockIdx.y];
   const T *X = static_cast<const T *>(ws_) + mt.x_off;
   const T *V = static_cast<const T *>(ws_) + mt.v_off;
+  const double *nm = norms + mt.p_off;
+  const int32_t *so = ord + mt.p_off;
   T *U = static_cast<T *>(u_) + mt.u_off;
   T *Vt = static_cast<T *>(vt_) + mt.vt_off;
   double *S = s_ + mt.s_off;
-  norms_and_order(X, mt.M, mt.N, sn, so);
   const int64_t r = mt.N;
-  for (int64_t k = threadIdx.x; k < r; k += kThreads) S[k] = sqrt(sn[so[k]]);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  const int64_t e0 = blockIdx.x * kThreads + threadIdx.x;
+  for (int64_t k = e0; k < r; k += stride) S[k] = sqrt(nm[so[k]]);
   if (!mt.tr) {
     // A = X_final V^H:  U (m x r) = X_col / sigma, Vt (r x n) = V^H
-    for (int64_t e = threadIdx.x; e < mt.m * r; e += kThreads) {
+    for (int64_t e = e0; e < mt.m * r; e += stride) {
       const int64_t i = e / r, k = e - i * r;
       const int32_t c = so[k];
-      const double sg = sqrt(sn[c]);
+      const double sg = sqrt(nm[c]);
       U[e] = sg > 0.0 ? scl_(1.0 / sg, X[int64_t(c) * mt.M + i]) : zero_<T>();
     }
-    for (int64_t e = threadIdx.x; e < r * mt.n; e += kThreads) {
+    for (int64_t e = e0; e < r * mt.n; e += stride) {
       const int64_t k = e / mt.n, j = e - k * mt.n;
       Vt[e] = conj_(V[int64_t(so[k]) * mt.N + j]);
     }
   } else {
     // A^H = X_final V^H  =>  A = V X_final^H:  U (m x r) = V, Vt (r x n) = (X_col / sigma)^H
-    for (int64_t e = threadIdx.x; e < mt.m * r; e += kThreads) {
+    for (int64_t e = e0; e < mt.m * r; e += stride) {
       const int64_t i = e / r, k = e - i * r;
       U[e] = V[int64_t(so[k]) * mt.N + i];
     }
-    for (int64_t e = threadIdx.x; e < r * mt.n; e += kThreads) {
+    for (int64_t e = e0; e < r * mt.n; e += stride) {
       const int64_t k = e / mt.n, j = e - k * mt.n;
       const int32_t c = so[k];
-      const double sg = sqrt(sn[c]);
+      const double sg = sqrt(nm[c]);
       Vt[e] = sg > 0.0 ? scl_(1.0 / sg, conj_(X[int64_t(c) * mt.M + j])) : zero_<T>();
     }
   }
@@ -715,7 +746,7 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
   }
   QLT_CHECK_ARG(gitems.size() < (size_t(1) << 31) && uitems.size() < (size_t(1) << 31),
                 "qlt_block_svd: too many row groups in one call");
-  DevBuf d_ws, d_mats, d_items, d_perm, d_flags, d_gitems, d_uitems, d_part, d_q, d_qflag;
+  DevBuf d_ws, d_mats, d_items, d_perm, d_flags, d_gitems, d_uitems, d_part, d_q, d_qflag, d_norms;
   QLT_HIP(hipMalloc(&d_ws.p, std::max<size_t>(1, static_cast<size_t>(ws_elems) * esz)));
   QLT_HIP(hipMalloc(&d_mats.p, sizeof(SvdMat) * mats.size()));
   QLT_HIP(hipMalloc(&d_items.p, sizeof(SvdItem) * std::max<size_t>(1, items.size())));
@@ -725,6 +756,7 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
   QLT_HIP(hipMalloc(&d_q.p, esz * kW * kW * static_cast<size_t>(std::max<int32_t>(1, pair_slots))));
   QLT_HIP(hipMalloc(&d_qflag.p, sizeof(int32_t) * static_cast<size_t>(std::max<int32_t>(1, pair_slots))));
   QLT_HIP(hipMalloc(&d_perm.p, sizeof(int32_t) * std::max<int64_t>(1, perm_elems)));
+  QLT_HIP(hipMalloc(&d_norms.p, sizeof(double) * std::max<int64_t>(1, perm_elems)));
   QLT_HIP(hipMalloc(&d_flags.p, sizeof(int32_t) * 2 * mats.size()));
   auto *active = static_cast<int32_t *>(d_flags.p);
   auto *rotated = active + mats.size();
@@ -741,6 +773,10 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
   auto *qflag = static_cast<int32_t *>(d_qflag.p);
   const auto *dm = static_cast<const SvdMat *>(d_mats.p);
   auto *perm = static_cast<int32_t *>(d_perm.p);
+  auto *norms = static_cast<double *>(d_norms.p);
+  int64_t max_n = 1;
+  for (const SvdMat &mt : mats) max_n = std::max(max_n, mt.N);
+  const dim3 norms_grid(static_cast<unsigned>((max_n + kWaves - 1) / kWaves), static_cast<unsigned>(n_mats));
   const unsigned gx = static_cast<unsigned>(std::min<int64_t>(1024, std::max<int64_t>(1, (max_elems + kThreads - 1) / kThreads)));
   hipLaunchKernelGGL(k_svd_load<C>, dim3(gx, static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, a,
                      d_ws.p, perm);
@@ -753,8 +789,9 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
     QLT_HIP(hipMemcpyAsync(active, h_active.data(), sizeof(int32_t) * mats.size(), hipMemcpyHostToDevice, st));
     std::fill(h_rot.begin(), h_rot.end(), 0);
     QLT_HIP(hipMemcpyAsync(rotated, h_rot.data(), sizeof(int32_t) * mats.size(), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_svd_sort<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm,
-                       d_ws.p, perm, active);
+    hipLaunchKernelGGL(k_svd_norms<C>, norms_grid, dim3(kThreads), 0, st, dm, d_ws.p, active, 0, norms);
+    hipLaunchKernelGGL(k_svd_sort, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, norms, perm,
+                       active, 0);
     for (int32_t r = 0; r < max_players - 1; ++r) {
       hipLaunchKernelGGL(k_svd_gram<C>, dim3(static_cast<unsigned>(gitems.size())), dim3(kThreads), 0, st, dm, gi,
                          r, d_ws.p, perm, active, d_part.p);
@@ -773,8 +810,12 @@ qlt_status svd_impl(int64_t n_mats, const int64_t *rows, const int64_t *cols, co
       any = any || h_active[i];
     }
   }
-  hipLaunchKernelGGL(k_svd_finish<C>, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, d_ws.p, u, s,
-                     vt);
+  // final order of every matrix (into the column-order buffer), then U, S, V^H
+  hipLaunchKernelGGL(k_svd_norms<C>, norms_grid, dim3(kThreads), 0, st, dm, d_ws.p, active, 1, norms);
+  hipLaunchKernelGGL(k_svd_sort, dim3(static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, norms, perm, active,
+                     1);
+  hipLaunchKernelGGL(k_svd_finish<C>, dim3(gx, static_cast<unsigned>(n_mats)), dim3(kThreads), 0, st, dm, d_ws.p,
+                     norms, perm, u, s, vt);
   QLT_HIP(hipGetLastError());
   QLT_HIP(hipStreamSynchronize(st));
   if (sweeps_out != nullptr) *sweeps_out = sweeps;
