@@ -85,6 +85,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+__device__ __forceinline__ double shfl_(double v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ cplx shfl_(cplx v, int src) { return {__shfl(v.x, src), __shfl(v.y, src)}; }
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
@@ -94,7 +96,6 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kB = 16;          // columns per Jacobi block
 constexpr int kW = 2 * kB;      // panel width
-constexpr int kRC = 16;         // rows per staged panel chunk
 constexpr int kRG = 64;         // rows per Gram / update workgroup
 constexpr int kMaxSweeps = 40;
 // Sweeps of the pair's Jacobi eigen-solve per visit: the outer sweeps repeat
@@ -130,14 +131,19 @@ struct SvdRowItem {
   int32_t kind;
 };
 
-// Circle-method pairing: `players` even, round r in [0, players - 1).
+// Circle-method pairing: `players` even, round r in [0, players - 1), pair
+// k in [0, players / 2).  Both sums lie in [0, 2 (players - 1)), so one
+// conditional subtraction reduces them modulo players - 1.
 __device__ __forceinline__ void circle_pair(int players, int r, int k, int &a, int &b) {
+  const int m = players - 1;
   if (k == 0) {
-    a = players - 1;
+    a = m;
     b = r;
   } else {
-    a = (r + k) % (players - 1);
-    b = (r - k + players - 1) % (players - 1);
+    a = r + k;
+    if (a >= m) a -= m;
+    b = r - k + m;
+    if (b >= m) b -= m;
   }
 }
 
@@ -233,12 +239,37 @@ __device__ __forceinline__ bool pair_cols(const SvdMat &mt, int32_t round, int32
 
 // Round phase 1: the partial Gram matrix P^H P of one kRG-row group of the
 // pair's panel P = [X_I X_J], stored w x w with row stride kW.
+// Stage rows rb .. re of the panel's w columns (column c = X column sCol[c])
+// into LDS as real and imaginary planes sP[part][c][row], zero beyond the
+// rows and columns.
+template <bool C, typename T>
+__device__ __forceinline__ void stage_panel(const T *base, int64_t rows, int64_t rb, int64_t re, int w,
+                                            const int32_t *sCol, double (*sP)[kW][kRG + 1]) {
+  for (int idx = threadIdx.x; idx < kW * kRG; idx += kThreads) {
+    const int c = idx / kRG, rr = idx - c * kRG;
+    const int64_t row = rb + rr;
+    T v = zero_<T>();
+    if (c < w && row < re) v = base[int64_t(sCol[c]) * rows + row];
+    if constexpr (C) {
+      sP[0][c][rr] = v.x;
+      sP[1][c][rr] = v.y;
+    } else {
+      sP[0][c][rr] = v;
+    }
+  }
+}
+
+// One row group's part of the pair's Gram matrix, G = P^H P over rows
+// g kRG .. (g + 1) kRG - 1, on v_mfma_f64_16x16x4f64: wave wv computes the
+// 16 x 16 tile (wv / 2, wv % 2) of the 32 x 32 result over 16 k-steps of 4
+// rows.  Complex: Re G = Pr^T Pr + Pi^T Pi, Im G = Pr^T Pi - Pi^T Pr.
 template <bool C>
 __global__ void __launch_bounds__(kThreads) k_svd_gram(const SvdMat *mats, const SvdRowItem *items, int32_t round,
                                                         const void *ws_, const int32_t *perm, const int32_t *active,
                                                         void *part_) {
   using T = typename ElemOf<C>::T;
-  __shared__ T sP[kRC][kW + 1];
+  constexpr int PARTS = C ? 2 : 1;
+  __shared__ double sP[PARTS][kW][kRG + 1];
   __shared__ int32_t sCol[kW];
   const SvdRowItem it = items[blockIdx.x];
   if (!active[it.mat]) return;
@@ -249,32 +280,33 @@ __global__ void __launch_bounds__(kThreads) k_svd_gram(const SvdMat *mats, const
   const T *X = static_cast<const T *>(ws_) + mt.x_off;
   __syncthreads();
   const int64_t rb = int64_t(it.g) * kRG, re = min<int64_t>(mt.M, rb + kRG);
-  T acc[4] = {zero_<T>(), zero_<T>(), zero_<T>(), zero_<T>()};
-  for (int64_t r0 = rb; r0 < re; r0 += kRC) {
-    for (int idx = tid; idx < kRC * w; idx += kThreads) {
-      const int c = idx / kRC, rr = idx - c * kRC;
-      const int64_t row = r0 + rr;
-      sP[rr][c] = row < re ? X[int64_t(sCol[c]) * mt.M + row] : zero_<T>();
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + u * kThreads;
-      if (e < w * w) {
-        const int p = e / w, q = e - p * w;
-        T s = acc[u];
+  stage_panel<C>(X, mt.M, rb, re, w, sCol, sP);
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int p0 = 16 * (wv >> 1), q0 = 16 * (wv & 1);
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  v4d acc = {0.0, 0.0, 0.0, 0.0}, acc_i = {0.0, 0.0, 0.0, 0.0};
+  if (p0 < w && q0 < w) {
 #pragma unroll 4
-        for (int rr = 0; rr < kRC; ++rr) s = add_(s, cmul_(sP[rr][p], sP[rr][q]));
-        acc[u] = s;
+    for (int ks = 0; ks < kRG; ks += 4) {
+      const double ar = sP[0][p0 + lr][ks + lk], br = sP[0][q0 + lr][ks + lk];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc, 0, 0, 0);
+      if constexpr (C) {
+        const double ai = sP[1][p0 + lr][ks + lk], bi = sP[1][q0 + lr][ks + lk];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc, 0, 0, 0);
+        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i, 0, 0, 0);
+        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acc_i, 0, 0, 0);
       }
     }
-    __syncthreads();
   }
   T *part = static_cast<T *>(part_) + (mt.gram0 + int64_t(it.k) * mt.ng + it.g) * (kW * kW);
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int e = tid + u * kThreads;
-    if (e < w * w) part[(e / w) * kW + e % w] = acc[u];
+  for (int v = 0; v < 4; ++v) {
+    const int p = p0 + 4 * v + lk, q = q0 + lr;
+    if (p < w && q < w) {
+      if constexpr (C) part[p * kW + q] = cplx{acc[v], acc_i[v]};
+      else part[p * kW + q] = acc[v];
+    }
   }
 }
 
@@ -344,8 +376,9 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
   // (the same operations, in the same order, as a column pass followed by a
   // row pass), and writes the result into the other G buffer; it also
   // rotates rows 2 bi, 2 bi + 1 of Q in the columns of pair bj, in place.
-  // Every thread computes the rotations it needs from the round's G itself,
-  // so a round takes one barrier.
+  // Every wave computes the round's rotations from the round's G, pair k in
+  // lane k (and its copies k + 16, ...), and each lane takes those of pairs
+  // bi and bj by shuffle, so a round takes one barrier.
   for (int e = tid; e < w * w; e += kThreads) {
     const int p = e / w, q = e - p * w;
     sQ[p][q] = (p == q) ? one_<T>() : zero_<T>();
@@ -385,15 +418,19 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
   for (int sweep = 0; sweep < kInnerSweeps; ++sweep) {
     int any = 0;
     for (int r = 0; r < P2 - 1; ++r) {
+      const T(*G)[kW + 1] = sG[cur];
+      const int kr = (tid & 63) % (kW / 2);
+      int rp = 0, rq = 0;
+      double rc = 1.0, rs = 0.0;
+      T rem = one_<T>();
+      const bool ron = kr < P2 / 2 && rotation(G, r, kr, rp, rq, rc, rs, rem);
+      any |= ron ? 1 : 0;
+      const int pi = __shfl(rp, bi), qi = __shfl(rq, bi), pj = __shfl(rp, bj), qj = __shfl(rq, bj);
+      const bool oni = __shfl(ron ? 1 : 0, bi) != 0, onj = __shfl(ron ? 1 : 0, bj) != 0;
+      const double ci = __shfl(rc, bi), si = __shfl(rs, bi), cj = __shfl(rc, bj), sj = __shfl(rs, bj);
+      const T emi = shfl_(rem, bi), emj = shfl_(rem, bj);
       if (blk_on) {
-        const T(*G)[kW + 1] = sG[cur];
         T(*Gn)[kW + 1] = sG[cur ^ 1];
-        int pi, qi, pj, qj;
-        double ci, si, cj, sj;
-        T emi, emj;
-        const bool oni = rotation(G, r, bi, pi, qi, ci, si, emi);
-        const bool onj = rotation(G, r, bj, pj, qj, cj, sj, emj);
-        any |= oni ? 1 : 0;
         const int rows[2] = {pi, qi}, cols[2] = {pj, qj};
         T x[2][2];
 #pragma unroll
@@ -454,13 +491,18 @@ __global__ void __launch_bounds__(kThreads) k_svd_solve(const SvdMat *mats, cons
 
 // Round phase 3: P <- P Q on one kRG-row group of the pair's columns of X
 // (kind 0) or of V (kind 1).
+// P <- P Q (and V <- V Q) for one row group of the pair, on
+// v_mfma_f64_16x16x4f64: wave wv computes rows 16 wv .. 16 wv + 15 of the
+// kRG-row group, in two 16-column tiles, over k-steps of 4 of the pair's w
+// columns.  Complex: Re = Pr Qr - Pi Qi, Im = Pr Qi + Pi Qr.
 template <bool C>
 __global__ void __launch_bounds__(kThreads) k_svd_update(const SvdMat *mats, const SvdRowItem *items, int32_t round,
                                                           void *ws_, const int32_t *perm, const int32_t *active,
                                                           const void *q_, const int32_t *qflag) {
   using T = typename ElemOf<C>::T;
-  __shared__ T sP[kRC][kW + 1];
-  __shared__ T sQ[kW][kW + 1];
+  constexpr int PARTS = C ? 2 : 1;
+  __shared__ double sP[PARTS][kW][kRG + 1];
+  __shared__ double sQ[PARTS][kW][kW];
   __shared__ int32_t sCol[kW];
   const SvdRowItem it = items[blockIdx.x];
   if (!active[it.mat]) return;
@@ -470,32 +512,56 @@ __global__ void __launch_bounds__(kThreads) k_svd_update(const SvdMat *mats, con
   if (!qflag[mt.pair0 + it.k]) return;
   const int tid = threadIdx.x;
   const T *Q = static_cast<const T *>(q_) + int64_t(mt.pair0 + it.k) * (kW * kW);
-  for (int e = tid; e < w * w; e += kThreads) {
-    const int p = e / w, q = e - p * w;
-    sQ[p][q] = Q[p * kW + q];
+  for (int e = tid; e < kW * kW; e += kThreads) {
+    const int p = e / kW, q = e - p * kW;
+    const T v = (p < w && q < w) ? Q[p * kW + q] : zero_<T>();
+    if constexpr (C) {
+      sQ[0][p][q] = v.x;
+      sQ[1][p][q] = v.y;
+    } else {
+      sQ[0][p][q] = v;
+    }
   }
   T *base = static_cast<T *>(ws_) + (it.kind ? mt.v_off : mt.x_off);
   const int64_t rows = it.kind ? mt.N : mt.M;
   const int64_t rb = int64_t(it.g) * kRG, re = min<int64_t>(rows, rb + kRG);
   __syncthreads();
-  for (int64_t r0 = rb; r0 < re; r0 += kRC) {
-    for (int idx = tid; idx < kRC * w; idx += kThreads) {
-      const int c = idx / kRC, rr = idx - c * kRC;
-      const int64_t row = r0 + rr;
-      sP[rr][c] = row < re ? base[int64_t(sCol[c]) * rows + row] : zero_<T>();
-    }
-    __syncthreads();
-    for (int idx = tid; idx < kRC * w; idx += kThreads) {
-      const int j = idx / kRC, rr = idx - j * kRC;
-      const int64_t row = r0 + rr;
-      if (row < re) {
-        T s = zero_<T>();
-        for (int c = 0; c < w; ++c) s = add_(s, mul_(sP[rr][c], sQ[c][j]));
-        base[int64_t(sCol[j]) * rows + row] = s;
+  stage_panel<C>(base, rows, rb, re, w, sCol, sP);
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  v4d acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  v4d acc_i[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  if (rb + 16 * wv < re) {
+    for (int ks = 0; ks < w; ks += 4) {
+      const double ar = sP[0][ks + lk][16 * wv + lr];
+      const double ai = C ? sP[PARTS - 1][ks + lk][16 * wv + lr] : 0.0;
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        const double br = sQ[0][ks + lk][16 * nt + lr];
+        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc[nt], 0, 0, 0);
+        if constexpr (C) {
+          const double bi = sQ[1][ks + lk][16 * nt + lr];
+          acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, acc[nt], 0, 0, 0);
+          acc_i[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i[nt], 0, 0, 0);
+          acc_i[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc_i[nt], 0, 0, 0);
+        }
       }
     }
-    __syncthreads();
   }
+  // every read of the rows is done (they sit in LDS): write the new columns
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = rb + 16 * wv + 4 * v + lk;
+      const int j = 16 * nt + lr;
+      if (row < re && j < w) {
+        T *dst = base + int64_t(sCol[j]) * rows + row;
+        if constexpr (C) *dst = cplx{acc[nt][v], acc_i[nt][v]};
+        else *dst = acc[nt][v];
+      }
+    }
 }
 
 // Singular values and vectors, grid (element groups, matrices): the columns
