@@ -62,22 +62,20 @@ class BlockMatrix:
     row_scts: list = field(default_factory=list)   # (lpart coords, offset, dim)
     col_scts: list = field(default_factory=list)   # (rpart coords, offset, dim)
     elems: list = field(default_factory=list)      # (i, j, block position in t)
-
-    def _sct(self, scts, coords, dim, is_row):
-        for k, (c, _, _) in enumerate(scts):
-            if c == coords:
-                return k
-        off = sum(d for _, _, d in scts)
-        scts.append((coords, off, dim))
-        if is_row:
-            self.rows += dim
-        else:
-            self.cols += dim
-        return len(scts) - 1
+    _row_pos: dict = field(default_factory=dict, repr=False)   # lpart coords -> row sector
+    _col_pos: dict = field(default_factory=dict, repr=False)
 
     def insert(self, m, n, blk_pos, lpart, rpart):
-        i = self._sct(self.row_scts, lpart, m, True)
-        j = self._sct(self.col_scts, rpart, n, False)
+        i = self._row_pos.get(lpart)
+        if i is None:
+            i = self._row_pos[lpart] = len(self.row_scts)
+            self.row_scts.append((lpart, self.rows, m))
+            self.rows += m
+        j = self._col_pos.get(rpart)
+        if j is None:
+            j = self._col_pos[rpart] = len(self.col_scts)
+            self.col_scts.append((rpart, self.cols, n))
+            self.cols += n
         self.elems.append((i, j, blk_pos))
 
 
@@ -87,24 +85,28 @@ def block_matrices(t: BlockSparseTensor, ldims: int, lqndiv) -> list:
     legs of (+qn for OUT, -qn for IN) (CalcDataBlkMatLeftQNFlux, :40-66);
     block matrices keyed by (left flux, right flux) in first-seen order."""
     div = tensor_div(t)
+    nb = t.n_blks
+    if nb == 0:
+        return []
+    coords = np.asarray(t.coords, dtype=np.int64).reshape(nb, -1)
+    dims = np.asarray(t.dims, dtype=np.int64).reshape(nb, -1)
+    lfl = np.zeros((nb, len(div)), dtype=np.int64)
+    for ax in range(ldims):
+        ix = t.indexes[ax]
+        lfl += ix.dir * np.asarray(ix.qns, dtype=np.int64).reshape(ix.nsec, -1)[coords[:, ax]]
+    rfl = np.asarray(div, dtype=np.int64)[None, :] - lfl
+    ms = np.prod(dims[:, :ldims], axis=1).tolist()
+    ns = np.prod(dims[:, ldims:], axis=1).tolist()
+    lfl, rfl, co = lfl.tolist(), rfl.tolist(), coords.tolist()
     mats: list[BlockMatrix] = []
-    for b in range(t.n_blks):
-        co = tuple(int(x) for x in t.coords[b])
-        dims = t.dims[b]
-        lflux = tuple([0] * len(div))
-        for ax in range(ldims):
-            ix = t.indexes[ax]
-            lflux = _qn_add(lflux, tuple(ix.dir * v for v in ix.qns[co[ax]]))
-        rflux = _qn_sub(div, lflux)
-        m = int(np.prod(dims[:ldims]))
-        n = int(np.prod(dims[ldims:]))
-        for bm in mats:
-            if bm.lflux == lflux and bm.rflux == rflux:
-                break
-        else:
-            bm = BlockMatrix(lflux, rflux, _qn_sub(tuple(lqndiv), lflux))
+    by_flux: dict = {}
+    for b in range(nb):
+        key = (tuple(lfl[b]), tuple(rfl[b]))
+        bm = by_flux.get(key)
+        if bm is None:
+            bm = by_flux[key] = BlockMatrix(key[0], key[1], _qn_sub(tuple(lqndiv), key[0]))
             mats.append(bm)
-        bm.insert(m, n, b, co[:ldims], co[ldims:])
+        bm.insert(ms[b], ns[b], b, tuple(co[b][:ldims]), tuple(co[b][ldims:]))
     return mats
 
 
@@ -167,33 +169,29 @@ def truncate(svals, trunc_err, dmin, dmax):
     """CalcTruncedSVInfo_ (ten_svd.h:281-372).  svals: list of per-matrix
     descending singular values.  Returns (kept count per matrix, actual
     truncation error, D).  Zero singular values are never counted."""
-    info = [(float(s), mi, i) for mi, sv in enumerate(svals) for i, s in enumerate(sv) if s != 0.0]
-    total_n = len(info)
-    kept = [0] * len(svals)
+    nm = len(svals)
+    lens = [len(sv) for sv in svals]
+    s = np.concatenate([np.asarray(sv, dtype=np.float64) for sv in svals]) if sum(lens) else np.zeros(0)
+    mi = np.repeat(np.arange(nm), lens)
+    nz = s != 0.0
+    s, mi = s[nz], mi[nz]
+    total_n = len(s)
     if total_n <= dmin:
-        for _, mi, _ in info:
-            kept[mi] += 1
-        return kept, 0.0, total_n
-    info.sort(key=lambda x: -x[0])
-    sq = [x[0] * x[0] for x in info]
-    total = sum(sq)
+        return np.bincount(mi, minlength=nm).tolist(), 0.0, total_n
+    # descending, ties in (matrix, position) order; c[k] = sum of the k
+    # largest squares, summed in that order
+    order = np.argsort(-s, kind="stable")
+    sq = s[order] * s[order]
+    c = np.concatenate([[0.0], np.cumsum(sq)])
+    total = c[-1]
     target = total * (1.0 - trunc_err)
-    kd = dmin
-    ksum = sum(sq[:kd])
-    while True:
-        if ksum > target:
-            break
-        nxt = kd + 1
-        if nxt > total_n:
-            break
-        nsum = ksum + sq[nxt - 1]
-        if nxt > dmax:
-            break
-        kd, ksum = nxt, nsum
-    err = 1.0 - ksum / total
-    for _, mi, _ in info[:kd]:
-        kept[mi] += 1
-    return kept, (err if err > 0 else 0.0), kd
+    # the smallest kd >= dmin whose kept weight exceeds the target, at most
+    # min(total_n, dmax)
+    limit = min(total_n, dmax)
+    over = np.nonzero(c[dmin:limit + 1] > target)[0]
+    kd = dmin + int(over[0]) if len(over) else limit
+    err = 1.0 - c[kd] / total
+    return np.bincount(mi[order[:kd]], minlength=nm).tolist(), (float(err) if err > 0 else 0.0), kd
 
 
 @dataclass
