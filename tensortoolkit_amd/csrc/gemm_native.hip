@@ -45,6 +45,20 @@ __device__ __forceinline__ double2 ld_elem(const E *p) {
     return double2{static_cast<double>(*p), 0.0};
   }
 }
+// The same load with the non-temporal hint, for streams read exactly once
+// (FP64 storage; the single-precision types take the plain load).
+template <typename E, bool CPLX>
+__device__ __forceinline__ double2 ld_elem_stream(const E *p) {
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  if constexpr (CPLX && sizeof(E) == 8) {
+    const v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p));
+    return double2{v[0], v[1]};
+  } else if constexpr (!CPLX && sizeof(E) == 8) {
+    return double2{__builtin_nontemporal_load(reinterpret_cast<const double *>(p)), 0.0};
+  } else {
+    return ld_elem<E, CPLX>(p);
+  }
+}
 template <typename E, bool CPLX>
 __device__ __forceinline__ void st_elem(E *p, double re, double im) {
   if constexpr (CPLX) {
@@ -774,14 +788,15 @@ __global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict_
       for (int c = 0; c < kPanelCols; ++c) roff[c][q] += d * pt.cstr[c][l];
     }
   }
-  // every A' load is issued before anything waits on memory
+  // every A' load is issued before anything waits on memory; each A'
+  // element is read once, so the loads carry the non-temporal hint
   double av[kPanelCols][RPT], av_i[kPanelCols][RPT];
 #pragma unroll
   for (int c = 0; c < kPanelCols; ++c)
     if (c < n_cols) {
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        const double2 v = ld_elem<E, CPLX>(A + (pt.cbase[c] + roff[c][q]) * P);
+        const double2 v = ld_elem_stream<E, CPLX>(A + (pt.cbase[c] + roff[c][q]) * P);
         av[c][q] = v.x;
         av_i[c][q] = v.y;
       }
