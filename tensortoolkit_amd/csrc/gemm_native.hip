@@ -757,7 +757,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict_
                                                     double beta_r, double beta_i) {
   constexpr int P = CPLX ? 2 : 1;
   constexpr int RPT = kPanelRpt;
+  static_assert(kPanelRows == 512 && RPT == 2, "pair staging: 4 loads per thread");
   __shared__ double s_br[kPanelMaxTerms], s_bi[kPanelMaxTerms];
+  __shared__ double s_pa[2][kPanelRows], s_pai[CPLX ? 2 : 1][kPanelRows];
   __shared__ int64_t s_ocoff[kPanelMaxOuts];
   __shared__ int32_t s_on[kPanelMaxOuts], s_ot0[kPanelMaxOuts], s_omask[kPanelMaxOuts];
   static_assert(kPanelMaxTerms <= 128 && kPanelMaxOuts <= 128, "one staging thread per term and per output");
@@ -789,11 +791,33 @@ __global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict_
     }
   }
   // every A' load is issued before anything waits on memory; each A'
-  // element is read once, so the loads carry the non-temporal hint
+  // element is read once, so the loads carry the non-temporal hint.  A pair
+  // of columns that are adjacent elements of one A block (pt.pair1 - 1 and
+  // pt.pair1) is loaded as (row, column) = (e / 2, e % 2) over the lanes, so
+  // that one instruction reads both columns of 32 consecutive rows, and
+  // reaches its rows' threads through LDS.
+  const int pc = pt.pair1 - 1;   // first column of the pair, or -1
   double av[kPanelCols][RPT], av_i[kPanelCols][RPT];
+  if (pc >= 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rl = (tid >> 1) + 128 * j, cc = tid & 1;
+      const int row = pt.r0 + rl;
+      int32_t x = row < pt.m ? row : 0, off = 0;
+#pragma unroll
+      for (int l = kPanelRowAxes - 1; l >= 0; --l) {
+        const int32_t qd = static_cast<int32_t>((static_cast<uint64_t>(x) * pt.rmag[l]) >> pt.rsh[l]);
+        off += (x - qd * pt.rdim[l]) * pt.cstr[pc][l];
+        x = qd;
+      }
+      const double2 v = ld_elem_stream<E, CPLX>(A + (pt.cbase[pc] + off + cc) * P);
+      s_pa[cc][rl] = v.x;
+      if (CPLX) s_pai[CPLX ? cc : 0][rl] = v.y;
+    }
+  }
 #pragma unroll
   for (int c = 0; c < kPanelCols; ++c)
-    if (c < n_cols) {
+    if (c < n_cols && (pc < 0 || (c != pc && c != pc + 1))) {
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const double2 v = ld_elem_stream<E, CPLX>(A + (pt.cbase[c] + roff[c][q]) * P);
@@ -817,6 +841,17 @@ __global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict_
     s_omask[o] = po.mask;
   }
   __syncthreads();
+  // the paired columns of this thread's rows
+#pragma unroll
+  for (int c = 0; c < kPanelCols; ++c)
+    if (pc >= 0 && (c == pc || c == pc + 1)) {
+      const int cc = c - pc;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        av[c][q] = s_pa[cc][tid + 256 * q];
+        av_i[c][q] = CPLX ? s_pai[CPLX ? cc : 0][tid + 256 * q] : 0.0;
+      }
+    }
 
   const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
   for (int o = 0; o < n_outs; ++o) {

@@ -622,6 +622,12 @@ qlt_status build_panels(qlt_plan *plan, std::vector<char> &in_panel) {
         pt.cbase[c] = a_off + host_split_offset(cols[c].second, g.cdim, g.cstr, L.a_nc);
         for (int l = 0; l < nr; ++l) pt.cstr[c][l] = g.rstr[l];
       }
+      pt.pair1 = 0;
+      for (int c = 0; c + 1 < static_cast<int>(cols.size()) && pt.pair1 == 0; ++c) {
+        bool adj = pt.cbase[c + 1] == pt.cbase[c] + 1;
+        for (int l = 0; l < kPanelRowAxes; ++l) adj = adj && pt.cstr[c][l] == pt.cstr[c + 1][l];
+        if (adj) pt.pair1 = c + 1;
+      }
       pt.m = m;
       pt.n_cols = static_cast<int32_t>(cols.size());
       pt.n_outs = static_cast<int32_t>(p_outs.size());

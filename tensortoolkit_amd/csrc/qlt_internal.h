@@ -239,7 +239,9 @@ struct PanelTile {
   int32_t n_terms;  // terms (PanelTerm records from term0)
   int32_t out0;
   int32_t term0;
-  int32_t pad;
+  // c + 1 when columns c and c + 1 are adjacent elements of one A block (the
+  // same row strides, bases one element apart; the first such c), else 0
+  int32_t pair1;
 };
 // Output column j of output block `seg`: one term per set bit c of `mask`
 // (A' column c of the panel), in ascending c, stored from term0 on (indices
