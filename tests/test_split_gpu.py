@@ -124,6 +124,36 @@ def test_split_pack_few_rows(forced_split, split_mode, dtype, k, m, a_transposed
 
 
 @pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_split_modes_alternate_on_one_plan(forced_split, dtype):
+    """One cached plan executed in BF16, int8, BF16 and native in turn: each
+    switch lays the split workspace out anew (slice planes of 5 BF16 or 6
+    int8 digits, their zero padding) and every result stays within 1e-12 of
+    the oracle; repeating a mode gives the same bits."""
+    rng = np.random.default_rng(4242)
+    a, b, a_ax, b_ax = random_case(rng, dtype=dtype, nq=1, max_rank=3, max_sectors=3, max_deg=40, min_deg=20,
+                                   allow_scalar=False)
+    plan = ContractionPlan(a, b, a_ax, b_ax)
+    raw = int(plan.info.c_raw_size)
+    da = capi.DeviceBuffer.from_numpy(a.data)
+    db = capi.DeviceBuffer.from_numpy(b.data)
+    exp = O.OraclePlan(a, b, a_ax, b_ax).execute()
+    got = {}
+    for step, mode in enumerate([capi.QLT_MODE_SPLIT_BF16, capi.QLT_MODE_SPLIT_I8, capi.QLT_MODE_SPLIT_BF16,
+                                 capi.QLT_MODE_NATIVE, capi.QLT_MODE_SPLIT_I8]):
+        _say(f"case: split mode switch {dtype} step {step} mode {mode}")
+        dc = capi.DeviceBuffer.from_numpy(np.full(max(1, raw), np.nan, dtype=a.np_dtype))
+        plan.execute(a.dtype, mode, da.ptr, db.ptr, dc.ptr)
+        capi.check(capi.lib().qlt_device_sync())
+        c = plan.output_tensor(a.dtype)
+        c.data = dc.to_numpy(a.np_dtype, raw)
+        assert per_block_rel_err(c, exp) <= TOL, (step, mode)
+        if mode in got:
+            np.testing.assert_array_equal(c.data, got[mode])
+        got[mode] = c.data.copy()
+        dc.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
 def test_split_k_beyond_int8_flush_interval(forced_split, split_mode, dtype):
     """k = 8500 crosses the int8 form's 8192-k flush of its int32 digit
     levels; entries in [0.5, 1) of one sign make every level sum grow as fast
