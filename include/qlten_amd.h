@@ -213,7 +213,8 @@ int32_t qlt_get_row_panels(void);
  * 1: 32x64, 2: 64x32, 3: 32x32; -1: the built-in rule, which takes 32 in a
  * dimension when that pads at most 9/10 as much as 64).  slice_target[c] is the
  * workgroup count that k-slicing of tile class c aims at (0: the built-in
- * 2048; classes 4 and 5 are the narrow-output row classes).  A plan keeps
+ * 512 for the MFMA classes 0-3, 2048 for classes 4 and 5, the narrow-output
+ * row classes).  A plan keeps
  * the tuning in effect when it was built. */
 #define QLT_TUNE_BUCKETS 16
 #define QLT_TILE_CLASSES 6

@@ -687,7 +687,8 @@ qlt_status build_panels(qlt_plan *plan, std::vector<char> &in_panel) {
 }
 
 // k-slicing of long tiles (MFMA and row classes alike).  Per class, a step budget Q = max(kMinSliceChunks,
-// ceil(total steps / T)) is set, T = kSliceTargetWGs unless tuned (qlt_tuning); a tile of w > Q steps is cut into
+// ceil(total steps / T)) is set, T = kSliceTargetWGsMfma (MFMA classes) or kSliceTargetWGs (row classes)
+// unless tuned (qlt_tuning); a tile of w > Q steps is cut into
 // ceil(w / Q) (at most kMaxSlices) slices of nearly equal length along its
 // (task, k-chunk) sequence, each writing an unscaled partial sum that a
 // reduction kernel adds in slice order.  Classes whose tiles are many and even
@@ -695,6 +696,9 @@ qlt_status build_panels(qlt_plan *plan, std::vector<char> &in_panel) {
 // output) spread over the chip.  Slices of >= kMinSliceChunks steps keep the
 // partial-sum traffic small against the slice's MFMA work.
 constexpr int64_t kSliceTargetWGs = 2048;
+// The MFMA classes aim lower: fewer, longer slices (and less partial-sum
+// traffic) measured faster on every matvec chain here than 2048.
+constexpr int64_t kSliceTargetWGsMfma = 512;
 constexpr int64_t kMinSliceChunks = 8;
 constexpr int64_t kMaxSlices = 64;
 
@@ -725,7 +729,8 @@ qlt_status slice_long_tiles(qlt_plan *plan) {
       if (in_panel.empty() || !in_panel[static_cast<size_t>(h[i].seg)]) total += seg_steps[static_cast<size_t>(h[i].seg)];
     // tuned workgroup target of the class (qlt_tuning; -1: unsliced)
     const bool unsliced = plan->slice_target[cls] < 0;
-    const int64_t target = plan->slice_target[cls] > 0 ? plan->slice_target[cls] : kSliceTargetWGs;
+    const int64_t target = plan->slice_target[cls] > 0 ? plan->slice_target[cls]
+                                                       : (is_mfma_class(cls) ? kSliceTargetWGsMfma : kSliceTargetWGs);
     const int64_t q = std::max(kMinSliceChunks, (total + target - 1) / target);
     std::vector<Tile> cl;
     int64_t parts = 0, n_red = 0;
