@@ -53,25 +53,33 @@ __global__ __launch_bounds__(256) void k_basis_dot_partial(int64_t n, int32_t k_
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   const int64_t first = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   for (int i0 = 0; i0 < m; i0 += kGroup) {
+    const int nq = min(kGroup, m - i0);
     double ar[kGroup], ai[kGroup];
 #pragma unroll
     for (int q = 0; q < kGroup; ++q) ar[q] = ai[q] = 0.0;
     for (int64_t e = first; e < n; e += stride) {
+      // the group's loads go out together (a vector past the count reads
+      // the group's first one and is not used)
       if (CPLX) {
         const double2 y = reinterpret_cast<const double2 *>(w)[e];
+        double2 x[kGroup];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) x[q] = reinterpret_cast<const double2 *>(V)[(i0 + (q < nq ? q : 0)) * ldv + e];
 #pragma unroll
         for (int q = 0; q < kGroup; ++q) {
-          if (i0 + q < m) {
-            const double2 x = reinterpret_cast<const double2 *>(V)[(i0 + q) * ldv + e];
-            ar[q] += x.x * y.x + x.y * y.y;
-            ai[q] += x.x * y.y - x.y * y.x;
+          if (q < nq) {
+            ar[q] += x[q].x * y.x + x[q].y * y.y;
+            ai[q] += x[q].x * y.y - x[q].y * y.x;
           }
         }
       } else {
         const double y = w[e];
+        double x[kGroup];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) x[q] = V[(i0 + (q < nq ? q : 0)) * ldv + e];
 #pragma unroll
         for (int q = 0; q < kGroup; ++q)
-          if (i0 + q < m) ar[q] += V[(i0 + q) * ldv + e] * y;
+          if (q < nq) ar[q] += x[q] * y;
       }
     }
 #pragma unroll
@@ -124,18 +132,35 @@ __global__ __launch_bounds__(256) void k_basis_update(int64_t n, int32_t k_max, 
   __syncthreads();
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n; e += stride) {
+    // kGroup basis loads in flight at a time, the terms still added in order of i
     if (CPLX) {
       double2 y = reinterpret_cast<const double2 *>(w)[e];
-      for (int i = 0; i < m; ++i) {
-        const double2 x = reinterpret_cast<const double2 *>(V)[i * ldv + e];
-        const double cr = s_c[2 * i], ci = s_c[2 * i + 1];
-        y.x += cr * x.x - ci * x.y;
-        y.y += cr * x.y + ci * x.x;
+      for (int i0 = 0; i0 < m; i0 += kGroup) {
+        const int nq = min(kGroup, m - i0);
+        double2 x[kGroup];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) x[q] = reinterpret_cast<const double2 *>(V)[(i0 + (q < nq ? q : 0)) * ldv + e];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) {
+          if (q < nq) {
+            const double cr = s_c[2 * (i0 + q)], ci = s_c[2 * (i0 + q) + 1];
+            y.x += cr * x[q].x - ci * x[q].y;
+            y.y += cr * x[q].y + ci * x[q].x;
+          }
+        }
       }
       reinterpret_cast<double2 *>(w)[e] = y;
     } else {
       double y = w[e];
-      for (int i = 0; i < m; ++i) y += s_c[2 * i] * V[i * ldv + e];
+      for (int i0 = 0; i0 < m; i0 += kGroup) {
+        const int nq = min(kGroup, m - i0);
+        double x[kGroup];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q) x[q] = V[(i0 + (q < nq ? q : 0)) * ldv + e];
+#pragma unroll
+        for (int q = 0; q < kGroup; ++q)
+          if (q < nq) y += s_c[2 * (i0 + q)] * x[q];
+      }
       w[e] = y;
     }
   }
