@@ -59,6 +59,25 @@ __device__ __forceinline__ double2 ld_elem_stream(const E *p) {
     return ld_elem<E, CPLX>(p);
   }
 }
+// The stored form of one element as it sits in registers between its load and
+// its LDS staging: double2 / float2 (complex), double / float (real).  The
+// GEMM fetches keep this form and convert to FP64 only when staging, so a
+// single-precision load is not followed by a conversion inside its guard,
+// which would make every load wait for its data before the next one issues.
+template <typename E, bool CPLX>
+using raw_t = std::conditional_t<CPLX, std::conditional_t<sizeof(E) == 8, double2, float2>, E>;
+template <typename E, bool CPLX>
+__device__ __forceinline__ raw_t<E, CPLX> ld_raw(const E *p) {
+  return *reinterpret_cast<const raw_t<E, CPLX> *>(p);
+}
+__device__ __forceinline__ double re_of(double2 v) { return v.x; }
+__device__ __forceinline__ double re_of(float2 v) { return static_cast<double>(v.x); }
+__device__ __forceinline__ double re_of(double v) { return v; }
+__device__ __forceinline__ double re_of(float v) { return static_cast<double>(v); }
+__device__ __forceinline__ double im_of(double2 v) { return v.y; }
+__device__ __forceinline__ double im_of(float2 v) { return static_cast<double>(v.y); }
+__device__ __forceinline__ double im_of(double) { return 0.0; }
+__device__ __forceinline__ double im_of(float) { return 0.0; }
 template <typename E, bool CPLX>
 __device__ __forceinline__ void st_elem(E *p, double re, double im) {
   if constexpr (CPLX) {
@@ -66,6 +85,76 @@ __device__ __forceinline__ void st_elem(E *p, double re, double im) {
     else *reinterpret_cast<float2 *>(p) = float2{static_cast<float>(re), static_cast<float>(im)};
   } else {
     *p = static_cast<E>(re);
+  }
+}
+
+// Epilogue of one output element: C = alpha * x + beta * old, with `old` the
+// element's earlier value, loaded by the caller beforehand (not read when
+// beta == 0).  The epilogues load every old value of a thread first, at
+// addresses clamped into the block, so the loads are in flight together
+// rather than each waited on before its store.
+template <typename E, bool CPLX>
+__device__ __forceinline__ void st_scaled(E *cp, double xr, double xi, const raw_t<E, CPLX> &old, bool use_beta,
+                                          double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  if constexpr (CPLX) {
+    double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
+    if (use_beta) {
+      const double o_r = re_of(old), o_i = im_of(old);
+      yr += beta_r * o_r - beta_i * o_i;
+      yi += beta_r * o_i + beta_i * o_r;
+    }
+    st_elem<E, true>(cp, yr, yi);
+  } else {
+    double y = alpha_r * xr;
+    if (use_beta) y += beta_r * re_of(old);
+    st_elem<E, false>(cp, y, 0.0);
+  }
+}
+
+// Epilogue of an MFMA output tile: element (mt, nt, v) of a thread sits at
+// row r0 + 16 mt + 4 v, column c0 + 16 nt.  With beta == 0 the stores go out
+// back to back; otherwise every old value is loaded first (addresses clamped
+// to the block's first element) and the stores follow.  The two cases are
+// separate code: merged, the waits for the loads would also hold back the
+// stores of the beta == 0 case, since stores count in the same counter.
+// ACC is one 16 x 16 sub-tile's 4 values of a thread (v4d, or double[4]).
+template <bool CPLX, int BM, int BN, typename E, typename ACC>
+__device__ __forceinline__ void tile_epilogue(const Seg &seg, int r0, int c0, const ACC (&acc)[BM][BN],
+                                              const ACC (&acc_i)[BM][BN], E *__restrict__ C, double alpha_r,
+                                              double alpha_i, double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
+  auto store_all = [&](auto with_beta, const raw_t<E, CPLX>(&old)[BM][BN][4]) {
+    constexpr bool WB = decltype(with_beta)::value;
+#pragma unroll
+    for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + mt * 16 + 4 * v, col = c0 + nt * 16;
+          if (row < seg.m && col < seg.n) {
+            const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
+            st_scaled<E, CPLX>(C + P * idx, acc[mt][nt][v], CPLX ? acc_i[mt][nt][v] : 0.0, old[mt][nt][v], WB,
+                               alpha_r, alpha_i, beta_r, beta_i);
+          }
+        }
+  };
+  if (beta_r != 0.0 || beta_i != 0.0) {
+    raw_t<E, CPLX> old[BM][BN][4];
+#pragma unroll
+    for (int mt = 0; mt < BM; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < BN; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + mt * 16 + 4 * v, col = c0 + nt * 16;
+          const bool ok = row < seg.m && col < seg.n;
+          old[mt][nt][v] = ld_raw<E, CPLX>(C + P * (seg.c_off + (ok ? static_cast<int64_t>(row) * seg.n + col : 0)));
+        }
+    store_all(std::true_type{}, old);
+  } else {
+    const raw_t<E, CPLX> none[BM][BN][4] = {};
+    store_all(std::false_type{}, none);
   }
 }
 
@@ -144,7 +233,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   // (row i, col p at i * ncols + p) needs no decomposition at all.
   int32_t a_row[QA], b_col[QB];
   bool a_plain = false, b_plain = false;
-  double2 pa[QA], pb[QB];
+  raw_t<E, CPLX> pa[QA], pb[QB];
   // every task of this output block reads plain blocks: no descriptor reads
   const bool seg_plain = (seg.flags & (kSegAPlain | kSegBPlain)) == (kSegAPlain | kSegBPlain);
   auto task_rows = [&](const BlkGather &gA, const BlkGather &gB, const ExecTask &et) {
@@ -170,7 +259,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       const int32_t ac = (pk < kend) ? (a_plain ? pk : split_offset(pk, gA.cdim, gA.cstr, gA.cmag, gA.csh, L.a_nc)) : -1;
       const bool ok = (a_row[q] >= 0) && (ac >= 0);
       const int64_t e = et.a_off + a_row[q] + ac;
-      pa[q] = ok ? ld_elem<E, CPLX>(A + e * P) : double2{0.0, 0.0};
+      pa[q] = ok ? ld_raw<E, CPLX>(A + e * P) : raw_t<E, CPLX>{};
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -178,19 +267,19 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
       const int32_t br = (pk < kend) ? (b_plain ? pk * seg.n : split_offset(pk, gB.rdim, gB.rstr, gB.rmag, gB.rsh, L.b_nr)) : -1;
       const bool ok = (b_col[q] >= 0) && (br >= 0);
       const int64_t e = et.b_off + br + b_col[q];
-      pb[q] = ok ? ld_elem<E, CPLX>(B + e * P) : double2{0.0, 0.0};
+      pb[q] = ok ? ld_raw<E, CPLX>(B + e * P) : raw_t<E, CPLX>{};
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
-      As[buf][0][a_p(q)][a_r(q)] = pa[q].x;
-      if (CPLX) As[buf][P - 1][a_p(q)][a_r(q)] = pa[q].y;
+      As[buf][0][a_p(q)][a_r(q)] = re_of(pa[q]);
+      if (CPLX) As[buf][P - 1][a_p(q)][a_r(q)] = im_of(pa[q]);
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      Bs[buf][0][b_p(q)][b_c(q)] = pb[q].x;
-      if (CPLX) Bs[buf][P - 1][b_p(q)][b_c(q)] = pb[q].y;
+      Bs[buf][0][b_p(q)][b_c(q)] = re_of(pb[q]);
+      if (CPLX) Bs[buf][P - 1][b_p(q)][b_c(q)] = im_of(pb[q]);
     }
   };
 
@@ -297,26 +386,26 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
 #pragma unroll
       for (int q = 0; q < QA; ++q) {
         const bool ok = aok[q] && (full || ap < kvalid);
-        pa[q] = ok ? ld_elem<E, CPLX>(aptr[q]) : double2{0.0, 0.0};
+        pa[q] = ok ? ld_raw<E, CPLX>(aptr[q]) : raw_t<E, CPLX>{};
         aptr[q] += a_adv;
       }
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
         const bool ok = bok[q] && (full || bp0 + BPS * q < kvalid);
-        pb[q] = ok ? ld_elem<E, CPLX>(bptr[q]) : double2{0.0, 0.0};
+        pb[q] = ok ? ld_raw<E, CPLX>(bptr[q]) : raw_t<E, CPLX>{};
         bptr[q] += b_adv;
       }
     };
     auto pstage = [&](int buf) {
 #pragma unroll
       for (int q = 0; q < QA; ++q) {
-        As[buf][0][ap][ar0 + 16 * q] = pa[q].x;
-        if (CPLX) As[buf][P - 1][ap][ar0 + 16 * q] = pa[q].y;
+        As[buf][0][ap][ar0 + 16 * q] = re_of(pa[q]);
+        if (CPLX) As[buf][P - 1][ap][ar0 + 16 * q] = im_of(pa[q]);
       }
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
-        Bs[buf][0][bp0 + BPS * q][bc] = pb[q].x;
-        if (CPLX) Bs[buf][P - 1][bp0 + BPS * q][bc] = pb[q].y;
+        Bs[buf][0][bp0 + BPS * q][bc] = re_of(pb[q]);
+        if (CPLX) Bs[buf][P - 1][bp0 + BPS * q][bc] = im_of(pb[q]);
       }
     };
     int t = tile.t_begin, kk = tile.k_begin, left = tile.n_chunks;
@@ -423,34 +512,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(
   }
 
   // ---- epilogue: C = alpha * acc + beta * C (C untouched when beta == 0) ----
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
-#pragma unroll
-  for (int mt = 0; mt < BM; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < BN; ++nt)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = i0 + wm + mt * 16 + 4 * v + lk;   // f64 16x16x4 output: row 4*reg + lane/16
-        const int col = j0 + wn + nt * 16 + lr;
-        if (row < seg.m && col < seg.n) {
-          const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
-          if (CPLX) {
-            const double xr = acc[mt][nt][v], xi = acc_i[mt][nt][v];
-            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            E *cp = C + 2 * idx;
-            if (use_beta) {
-              const double2 o = ld_elem<E, true>(cp);
-              yr += beta_r * o.x - beta_i * o.y;
-              yi += beta_r * o.y + beta_i * o.x;
-            }
-            st_elem<E, true>(cp, yr, yi);
-          } else {
-            double y = alpha_r * acc[mt][nt][v];
-            if (use_beta) y += beta_r * static_cast<double>(C[idx]);
-            st_elem<E, false>(C + idx, y, 0.0);
-          }
-        }
-      }
+  // f64 16x16x4 output: row 4*reg + lane/16
+  tile_epilogue<CPLX, BM, BN, E>(seg, i0 + wm + lk, j0 + wn + lr, acc, acc_i, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
 // Sum of the k-slices of one output tile, in slice order, then the usual
@@ -469,7 +532,6 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restr
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = (w >> 1) * (TMv / 2), wn = (w & 1) * (TNv / 2);
   const int lr = lane & 15, lk = lane >> 4;
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
   // slices outer, elements inner: every slice's loads are independent
   double sr[BM][BN][4], si[BM][BN][4];
 #pragma unroll
@@ -494,33 +556,7 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const ReduceItem *__restr
           if (CPLX) si[mt][nt][v] += pp[TMv * TNv + e];
         }
   }
-#pragma unroll
-  for (int mt = 0; mt < BM; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < BN; ++nt)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double xr = sr[mt][nt][v], xi = si[mt][nt][v];
-        const int row = i0 + wm + mt * 16 + 4 * v + lk;
-        const int col = j0 + wn + nt * 16 + lr;
-        if (row < seg.m && col < seg.n) {
-          const int64_t idx = seg.c_off + static_cast<int64_t>(row) * seg.n + col;
-          if (CPLX) {
-            double yr = alpha_r * xr - alpha_i * xi, yi = alpha_r * xi + alpha_i * xr;
-            E *cp = C + 2 * idx;
-            if (use_beta) {
-              const double2 o = ld_elem<E, true>(cp);
-              yr += beta_r * o.x - beta_i * o.y;
-              yi += beta_r * o.y + beta_i * o.x;
-            }
-            st_elem<E, true>(cp, yr, yi);
-          } else {
-            double y = alpha_r * xr;
-            if (use_beta) y += beta_r * static_cast<double>(C[idx]);
-            st_elem<E, false>(C + idx, y, 0.0);
-          }
-        }
-      }
+  tile_epilogue<CPLX, BM, BN, E>(seg, i0 + wm + lk, j0 + wn + lr, sr, si, C, alpha_r, alpha_i, beta_r, beta_i);
 }
 
 // Row-streaming kernel for narrow outputs (n <= NMAX): one thread per output
@@ -538,33 +574,40 @@ template <bool CPLX, int NMAX, int RPT, typename E>
 __device__ __forceinline__ void rows_epilogue(const Seg &seg, int r0, int tid, const double (&acc)[RPT][NMAX],
                                               const double (&acc_i)[RPT][NMAX], E *__restrict__ C,
                                               double alpha_r, double alpha_i, double beta_r, double beta_i) {
+  constexpr int P = CPLX ? 2 : 1;
   const int n = seg.n;
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
+  // beta == 0 and beta != 0 as separate code, as in tile_epilogue
+  auto store_all = [&](auto with_beta, const raw_t<E, CPLX>(&old)[RPT][NMAX]) {
+    constexpr bool WB = decltype(with_beta)::value;
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int row = r0 + tid + 256 * q;
-    if (row >= seg.m) continue;
-    const int64_t base = seg.c_off + static_cast<int64_t>(row) * n;
+    for (int q = 0; q < RPT; ++q) {
+      const int row = r0 + tid + 256 * q;
+      if (row >= seg.m) continue;
+      const int64_t base = seg.c_off + static_cast<int64_t>(row) * n;
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      if (j < n) {
-        if (CPLX) {
-          double yr = alpha_r * acc[q][j] - alpha_i * acc_i[q][j];
-          double yi = alpha_r * acc_i[q][j] + alpha_i * acc[q][j];
-          E *cp = C + 2 * (base + j);
-          if (use_beta) {
-            const double2 o = ld_elem<E, true>(cp);
-            yr += beta_r * o.x - beta_i * o.y;
-            yi += beta_r * o.y + beta_i * o.x;
-          }
-          st_elem<E, true>(cp, yr, yi);
-        } else {
-          double y = alpha_r * acc[q][j];
-          if (use_beta) y += beta_r * static_cast<double>(C[base + j]);
-          st_elem<E, false>(C + base + j, y, 0.0);
+      for (int j = 0; j < NMAX; ++j) {
+        if (j < n) {
+          st_scaled<E, CPLX>(C + P * (base + j), acc[q][j], CPLX ? acc_i[q][j] : 0.0, old[q][j], WB,
+                             alpha_r, alpha_i, beta_r, beta_i);
         }
       }
     }
+  };
+  if (beta_r != 0.0 || beta_i != 0.0) {
+    raw_t<E, CPLX> old[RPT][NMAX];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int row = r0 + tid + 256 * q;
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        const bool ok = row < seg.m && j < n;
+        old[q][j] = ld_raw<E, CPLX>(C + P * (seg.c_off + (ok ? static_cast<int64_t>(row) * n + j : 0)));
+      }
+    }
+    store_all(std::true_type{}, old);
+  } else {
+    const raw_t<E, CPLX> none[RPT][NMAX] = {};
+    store_all(std::false_type{}, none);
   }
 }
 
@@ -726,16 +769,22 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const ReduceItem *__restric
       acc[q][j] = 0.0;
       acc_i[q][j] = 0.0;
     }
+  // Each slice's slab holds the full RPT x NMAX layout, so every load is in
+  // bounds; columns j >= n are loaded and dropped, which keeps the loads of a
+  // slice in flight together.
   for (int s = 0; s < it.n_parts; ++s) {
     const double *pp = part + static_cast<int64_t>(it.part0 + s) * SZ;
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        if (j < seg.n) {
-          const int e = (q * NMAX + j) * 256 + tid;
-          acc[q][j] += pp[e];
-          if (CPLX) acc_i[q][j] += pp[256 * RPT * NMAX + e];
+        const int e = (q * NMAX + j) * 256 + tid;
+        const bool ok = j < seg.n;
+        const double x = pp[e];
+        acc[q][j] += ok ? x : 0.0;
+        if (CPLX) {
+          const double xi = pp[256 * RPT * NMAX + e];
+          acc_i[q][j] += ok ? xi : 0.0;
         }
       }
   }
@@ -857,58 +906,57 @@ __global__ __launch_bounds__(256) void k_panel_rows(const PanelTile *__restrict_
       }
     }
 
-  const bool use_beta = (beta_r != 0.0) || (beta_i != 0.0);
-  for (int o = 0; o < n_outs; ++o) {
-    const int64_t coff = s_ocoff[o];
-    const int32_t n = __builtin_amdgcn_readfirstlane(s_on[o]);
-    const int32_t mask = __builtin_amdgcn_readfirstlane(s_omask[o]);
-    int32_t t = __builtin_amdgcn_readfirstlane(s_ot0[o]);
-    double y[RPT], y_i[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      y[q] = 0.0;
-      y_i[q] = 0.0;
-    }
-    // the output's terms in ascending column order, which is their (task, k)
-    // order: the column loop is static, so no register array is indexed
-#pragma unroll
-    for (int c = 0; c < kPanelCols; ++c) {
-      if (!((mask >> c) & 1)) continue;
-      const double b = s_br[t], b_im = s_bi[t];
-      ++t;
+  // The output loop exists twice, for beta == 0 and beta != 0: with the
+  // loads of the old values inside a shared loop, every output column waited
+  // for the previous column's stores (stores count in the same counter).
+  auto out_loop = [&](auto with_beta) {
+    constexpr bool WB = decltype(with_beta)::value;
+    for (int o = 0; o < n_outs; ++o) {
+      const int64_t coff = s_ocoff[o];
+      const int32_t n = __builtin_amdgcn_readfirstlane(s_on[o]);
+      const int32_t mask = __builtin_amdgcn_readfirstlane(s_omask[o]);
+      int32_t t = __builtin_amdgcn_readfirstlane(s_ot0[o]);
+      double y[RPT], y_i[RPT];
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        if (CPLX) {
-          y[q] = fma(av[c][q], b, y[q]);
-          y[q] = fma(-av_i[c][q], b_im, y[q]);
-          y_i[q] = fma(av[c][q], b_im, y_i[q]);
-          y_i[q] = fma(av_i[c][q], b, y_i[q]);
-        } else {
-          y[q] = fma(av[c][q], b, y[q]);
-        }
+        y[q] = 0.0;
+        y_i[q] = 0.0;
       }
-    }
+      // the output's terms in ascending column order, which is their (task, k)
+      // order: the column loop is static, so no register array is indexed
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      if (!valid[q]) continue;
-      const int64_t e = coff + static_cast<int64_t>(pt.r0 + tid + 256 * q) * n;
-      if (CPLX) {
-        double yr = alpha_r * y[q] - alpha_i * y_i[q];
-        double yi = alpha_r * y_i[q] + alpha_i * y[q];
-        E *cp = C + 2 * e;
-        if (use_beta) {
-          const double2 old = ld_elem<E, true>(cp);
-          yr += beta_r * old.x - beta_i * old.y;
-          yi += beta_r * old.y + beta_i * old.x;
+      for (int c = 0; c < kPanelCols; ++c) {
+        if (!((mask >> c) & 1)) continue;
+        const double b = s_br[t], b_im = s_bi[t];
+        ++t;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          if (CPLX) {
+            y[q] = fma(av[c][q], b, y[q]);
+            y[q] = fma(-av_i[c][q], b_im, y[q]);
+            y_i[q] = fma(av[c][q], b_im, y_i[q]);
+            y_i[q] = fma(av_i[c][q], b, y_i[q]);
+          } else {
+            y[q] = fma(av[c][q], b, y[q]);
+          }
         }
-        st_elem<E, true>(cp, yr, yi);
-      } else {
-        double v = alpha_r * y[q];
-        if (use_beta) v += beta_r * static_cast<double>(C[e]);
-        st_elem<E, false>(C + e, v, 0.0);
+      }
+      raw_t<E, CPLX> old[RPT] = {};
+      if constexpr (WB) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          old[q] = ld_raw<E, CPLX>(C + P * (coff + (valid[q] ? static_cast<int64_t>(pt.r0 + tid + 256 * q) * n : 0)));
+      }
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        if (!valid[q]) continue;
+        const int64_t e = coff + static_cast<int64_t>(pt.r0 + tid + 256 * q) * n;
+        st_scaled<E, CPLX>(C + P * e, y[q], CPLX ? y_i[q] : 0.0, old[q], WB, alpha_r, alpha_i, beta_r, beta_i);
       }
     }
-  }
+  };
+  if (beta_r != 0.0 || beta_i != 0.0) out_loop(std::true_type{});
+  else out_loop(std::false_type{});
 }
 
 }  // namespace
