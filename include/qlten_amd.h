@@ -337,6 +337,12 @@ qlt_status qlt_scale_segments_noalloc(int32_t dtype, int64_t n_segs, const int64
  *                   0 for m <= i < k_max (c: 2 k_max doubles, device);
  *                   work: QLT_BASIS_WORK_DOUBLES(k_max) doubles, device.
  * qlt_basis_update: w += scale * sum_{i<m} c_i V_i (c complex pairs, device).
+ * qlt_basis_update_dot: qlt_basis_update, then qlt_basis_dot of the updated w
+ *                   into c_out (c_out must not alias c; work as for
+ *                   qlt_basis_dot), with bitwise the same results as the two
+ *                   calls; for m <= 16 in one pass over the basis (the second
+ *                   Gram-Schmidt pass's products formed while the first pass
+ *                   updates w).
  * qlt_lanczos_extend: with j = m - 1: alphas[j] = Re c[2j], betas[j] = |w|,
  *                   V_m = w / |w| (zero when |w| = 0) if m < k_max, then
  *                   m = m + 1 (when m < k_max).  work as for qlt_basis_dot.
@@ -346,6 +352,9 @@ qlt_status qlt_basis_dot(int32_t dtype, int64_t n, int32_t k_max, const int32_t 
                          const void *w, double *c, double *work, void *stream);
 qlt_status qlt_basis_update(int32_t dtype, int64_t n, int32_t k_max, const int32_t *count, double scale,
                             const double *c, const void *V, int64_t ldv, void *w, void *stream);
+qlt_status qlt_basis_update_dot(int32_t dtype, int64_t n, int32_t k_max, const int32_t *count, double scale,
+                                const double *c, const void *V, int64_t ldv, void *w, double *c_out, double *work,
+                                void *stream);
 qlt_status qlt_lanczos_extend(int32_t dtype, int64_t n, int32_t k_max, int32_t *count, const double *c,
                               const void *w, void *V, int64_t ldv, double *alphas, double *betas, double *work,
                               void *stream);

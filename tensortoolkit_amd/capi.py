@@ -160,6 +160,8 @@ _SIGS = {
                        C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_basis_update": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                           C.c_int64, C.c_void_p, C.c_void_p], C.c_int32),
+    "qlt_basis_update_dot": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                              C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_lanczos_extend": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                             C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
     "qlt_basis_select": ([C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64,

@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kParts = 512;          // partial sums per reduction (fixed grid)
 constexpr int kGroup = 8;            // basis vectors per pass over w
+constexpr int kFuseMax = 16;         // basis vectors whose values k_basis_update_dot keeps per element
 constexpr int32_t kMaxBasis = 1024;  // arena slots a call may address
 
 __device__ __forceinline__ int active_count(const int32_t *count, int32_t k_max) {
@@ -44,12 +45,16 @@ __device__ __forceinline__ double block_sum(double v, double *s_red) {
 
 // work[(i * kParts + b) * 2 + {0, 1}] = partial <V_i, w> of workgroup b
 // (conj(V_i) * w), for i < m.
+// With m <= skip_upto the kernel does nothing (k_basis_update_dot has
+// written the partials already).
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_basis_dot_partial(int64_t n, int32_t k_max, const int32_t *count,
                                                            const double *__restrict__ V, int64_t ldv,
-                                                           const double *__restrict__ w, double *work) {
+                                                           const double *__restrict__ w, double *work,
+                                                           int32_t skip_upto) {
   __shared__ double s_red[4];
   const int m = active_count(count, k_max);
+  if (m <= skip_upto) return;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   const int64_t first = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   for (int i0 = 0; i0 < m; i0 += kGroup) {
@@ -166,6 +171,118 @@ __global__ __launch_bounds__(256) void k_basis_update(int64_t n, int32_t k_max, 
   }
 }
 
+// The update of k_basis_update followed by the partial products of
+// k_basis_dot_partial on the updated w, in one pass over the basis when
+// m <= kFuseMax: each thread keeps its elements' basis values between the
+// two.  Same workgroup partition as k_basis_dot_partial (kParts workgroups),
+// same terms in the same order per element and per partial sum, so w and
+// the partials are bitwise those of the two kernels run one after the other.
+// With m > kFuseMax only the update is done here (k_basis_dot_partial then
+// runs on the updated w).
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_basis_update_dot(int64_t n, int32_t k_max, const int32_t *count,
+                                                          double scale, const double *__restrict__ c,
+                                                          const double *__restrict__ V, int64_t ldv,
+                                                          double *__restrict__ w, double *work) {
+  __shared__ double s_c[2 * kMaxBasis];
+  __shared__ double s_red[4];
+  const int m = active_count(count, k_max);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    s_c[2 * i] = scale * c[2 * i];
+    s_c[2 * i + 1] = scale * c[2 * i + 1];
+  }
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t first = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (m > kFuseMax) {
+    for (int64_t e = first; e < n; e += stride) {
+      if (CPLX) {
+        double2 y = reinterpret_cast<const double2 *>(w)[e];
+        for (int i0 = 0; i0 < m; i0 += kGroup) {
+          const int nq = min(kGroup, m - i0);
+          double2 x[kGroup];
+#pragma unroll
+          for (int q = 0; q < kGroup; ++q) x[q] = reinterpret_cast<const double2 *>(V)[(i0 + (q < nq ? q : 0)) * ldv + e];
+#pragma unroll
+          for (int q = 0; q < kGroup; ++q) {
+            if (q < nq) {
+              const double cr = s_c[2 * (i0 + q)], ci = s_c[2 * (i0 + q) + 1];
+              y.x += cr * x[q].x - ci * x[q].y;
+              y.y += cr * x[q].y + ci * x[q].x;
+            }
+          }
+        }
+        reinterpret_cast<double2 *>(w)[e] = y;
+      } else {
+        double y = w[e];
+        for (int i0 = 0; i0 < m; i0 += kGroup) {
+          const int nq = min(kGroup, m - i0);
+          double x[kGroup];
+#pragma unroll
+          for (int q = 0; q < kGroup; ++q) x[q] = V[(i0 + (q < nq ? q : 0)) * ldv + e];
+#pragma unroll
+          for (int q = 0; q < kGroup; ++q)
+            if (q < nq) y += s_c[2 * (i0 + q)] * x[q];
+        }
+        w[e] = y;
+      }
+    }
+    return;
+  }
+  double ar[kFuseMax], ai[kFuseMax];
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) ar[q] = ai[q] = 0.0;
+  for (int64_t e = first; e < n; e += stride) {
+    // all m basis values of the element go out together (a vector past the
+    // count reads vector 0 and is not used)
+    if (CPLX) {
+      double2 y = reinterpret_cast<const double2 *>(w)[e];
+      double2 x[kFuseMax];
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q) x[q] = reinterpret_cast<const double2 *>(V)[(q < m ? q : 0) * ldv + e];
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q) {
+        if (q < m) {
+          const double cr = s_c[2 * q], ci = s_c[2 * q + 1];
+          y.x += cr * x[q].x - ci * x[q].y;
+          y.y += cr * x[q].y + ci * x[q].x;
+        }
+      }
+      reinterpret_cast<double2 *>(w)[e] = y;
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q) {
+        if (q < m) {
+          ar[q] += x[q].x * y.x + x[q].y * y.y;
+          ai[q] += x[q].x * y.y - x[q].y * y.x;
+        }
+      }
+    } else {
+      double y = w[e];
+      double x[kFuseMax];
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q) x[q] = V[(q < m ? q : 0) * ldv + e];
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q)
+        if (q < m) y += s_c[2 * q] * x[q];
+      w[e] = y;
+#pragma unroll
+      for (int q = 0; q < kFuseMax; ++q)
+        if (q < m) ar[q] += x[q] * y;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) {
+    if (q < m) {
+      const double sr = block_sum(ar[q], s_red);
+      const double si = CPLX ? block_sum(ai[q], s_red) : 0.0;
+      if (threadIdx.x == 0) {
+        work[(static_cast<int64_t>(q) * kParts + blockIdx.x) * 2] = sr;
+        work[(static_cast<int64_t>(q) * kParts + blockIdx.x) * 2 + 1] = si;
+      }
+    }
+  }
+}
+
 // work[b] = partial sum of |w_e|^2 of workgroup b.
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_norm2_partial(int64_t n, const double *__restrict__ w, double *work) {
@@ -271,10 +388,38 @@ qlt_status qlt_basis_dot(int32_t dtype, int64_t n, int32_t k_max, const int32_t 
   const auto *vd = static_cast<const double *>(V);
   const auto *wd = static_cast<const double *>(w);
   if (dtype == QLT_C128)
-    hipLaunchKernelGGL(k_basis_dot_partial<true>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work);
+    hipLaunchKernelGGL(k_basis_dot_partial<true>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work,
+                       -1);
   else
-    hipLaunchKernelGGL(k_basis_dot_partial<false>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work);
+    hipLaunchKernelGGL(k_basis_dot_partial<false>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work,
+                       -1);
   hipLaunchKernelGGL(k_basis_dot_final, dim3(k_max), dim3(256), 0, st, k_max, count, work, c);
+  QLT_HIP(hipGetLastError());
+  return QLT_OK;
+}
+
+qlt_status qlt_basis_update_dot(int32_t dtype, int64_t n, int32_t k_max, const int32_t *count, double scale,
+                                const double *c, const void *V, int64_t ldv, void *w, double *c_out, double *work,
+                                void *stream) {
+  QLT_REQUIRE_INIT();
+  QLT_TRY(check_basis_args("qlt_basis_update_dot", dtype, n, k_max, ldv));
+  QLT_CHECK_ARG(count && V && w && c && c_out && work, "qlt_basis_update_dot: null pointer");
+  QLT_CHECK_ARG(c != c_out, "qlt_basis_update_dot: c and c_out must not alias");
+  hipStream_t st = as_stream(stream);
+  const auto *vd = static_cast<const double *>(V);
+  auto *wd = static_cast<double *>(w);
+  if (dtype == QLT_C128) {
+    hipLaunchKernelGGL(k_basis_update_dot<true>, dim3(kParts), dim3(256), 0, st, n, k_max, count, scale, c, vd, ldv,
+                       wd, work);
+    hipLaunchKernelGGL(k_basis_dot_partial<true>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work,
+                       kFuseMax);
+  } else {
+    hipLaunchKernelGGL(k_basis_update_dot<false>, dim3(kParts), dim3(256), 0, st, n, k_max, count, scale, c, vd, ldv,
+                       wd, work);
+    hipLaunchKernelGGL(k_basis_dot_partial<false>, dim3(kParts), dim3(256), 0, st, n, k_max, count, vd, ldv, wd, work,
+                       kFuseMax);
+  }
+  hipLaunchKernelGGL(k_basis_dot_final, dim3(k_max), dim3(256), 0, st, k_max, count, work, c_out);
   QLT_HIP(hipGetLastError());
   return QLT_OK;
 }

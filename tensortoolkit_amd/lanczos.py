@@ -176,10 +176,13 @@ class KrylovArena:
         lib, st, n, k, dt = capi.lib(), self.hv.stream, self.n, self.k_max, self.dt
         capi.check(lib.qlt_basis_select(dt, n, k, self.count.ptr, -1, self.V.ptr, n, self.x.ptr, st))
         self.hv.apply(self.x.ptr, self.w.ptr)
-        for c in (self.c1, self.c2):
-            capi.check(lib.qlt_basis_dot(dt, n, k, self.count.ptr, self.V.ptr, n, self.w.ptr, c.ptr,
-                                         self.work.ptr, st))
-            capi.check(lib.qlt_basis_update(dt, n, k, self.count.ptr, -1.0, c.ptr, self.V.ptr, n, self.w.ptr, st))
+        # pass 1: c1 = V^H w; w -= V c1 together with pass 2's c2 = V^H w
+        # (one read of the basis for both); pass 2: w -= V c2
+        capi.check(lib.qlt_basis_dot(dt, n, k, self.count.ptr, self.V.ptr, n, self.w.ptr, self.c1.ptr,
+                                     self.work.ptr, st))
+        capi.check(lib.qlt_basis_update_dot(dt, n, k, self.count.ptr, -1.0, self.c1.ptr, self.V.ptr, n, self.w.ptr,
+                                            self.c2.ptr, self.work.ptr, st))
+        capi.check(lib.qlt_basis_update(dt, n, k, self.count.ptr, -1.0, self.c2.ptr, self.V.ptr, n, self.w.ptr, st))
         capi.check(lib.qlt_lanczos_extend(dt, n, k, self.count.ptr, self.c1.ptr, self.w.ptr, self.V.ptr, n,
                                           self.alphas.ptr, self.betas.ptr, self.work.ptr, st))
 
